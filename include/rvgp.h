@@ -1,0 +1,187 @@
+/*
+ * rvgp.h -- C ABI of librvgp.so, the MI355X (gfx950) kernels behind the
+ * RVGP connection-Laplacian Gaussian-process path.
+ *
+ * Conventions (all entry points):
+ *   - Every pointer argument is DEVICE memory unless its comment says "host".
+ *     The caller allocates every buffer; the library never frees caller
+ *     memory.  Scratch comes from caller-provided workspace pointers whose
+ *     sizes are given by the matching *_ws_bytes() query.
+ *   - `stream` is a hipStream_t passed as void*; work is enqueued on it and
+ *     the call returns without synchronising unless the comment says so.
+ *   - Return value: RVGP_OK (0) or a negative status.  The Python layer maps
+ *     RVGP_EINVAL -> ValueError, RVGP_ERUNTIME / RVGP_EHIP -> RuntimeError,
+ *     RVGP_ENOMEM -> MemoryError, the exception types the reference raises
+ *     (RVGP/lib/ptu_dijkstra.pyx:68-82, 119-123, 358-359).
+ *   - Kernels that detect a data-dependent failure (a rank-deficient
+ *     neighbourhood, a non-positive-definite matrix) write a code into a
+ *     caller-provided device int `status`; the caller reads it after the
+ *     stream completes.
+ *   - Matrices are row-major; `ld*` is the row stride in elements.
+ *   - No global mutable state other than the thread-local error string.
+ *
+ * Each entry point names the reference interface it replaces (file:line,
+ * relative to the agosztolai/RVGP repository).
+ */
+#ifndef RVGP_H
+#define RVGP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RVGP_OK 0
+#define RVGP_EINVAL (-1)
+#define RVGP_ERUNTIME (-2)
+#define RVGP_ENOMEM (-3)
+#define RVGP_EHIP (-4)
+
+/* Device-side status codes written into `status` words. */
+#define RVGP_ST_OK 0
+#define RVGP_ST_RANK_DEFICIENT 1   /* a neighbourhood singular value < 1e-10 */
+#define RVGP_ST_SHORT_COMPONENT 2  /* fewer than K+1 nodes reachable */
+#define RVGP_ST_POOL_OVERFLOW 3    /* heap node pool too small (host sizing bug) */
+#define RVGP_ST_NOT_SPD 4          /* Cholesky met a non-positive pivot */
+
+/* Library version and last error message (thread-local, host). */
+int rvgp_version(void);
+const char* rvgp_last_error(void);
+
+/* ---------------------------------------------------------------------
+ * a1  Furthest point sampling  -- replaces geometry.furthest_point_sampling
+ *     (RVGP/geometry.py:126-162).
+ * ------------------------------------------------------------------- */
+
+/* diam2[0] = max over pairs of |x_i - x_j|^2 (the square of D.max(),
+ * geometry.py:142-144).  x: (n, D) f64.  diam2 must be zeroed by the
+ * caller (it is accumulated with an atomic max). */
+int rvgp_fps_diam2(const double* x, int n, int D, double* diam2, void* stream);
+
+/* Greedy farthest-point loop (geometry.py:146-160) in one persistent
+ * workgroup.  n_iter = N, or n when N is None.  If use_spacing != 0 the loop
+ * stops at the first i with lambdas[i]/sqrt(diam2) < spacing and writes i to
+ * n_sel[0]; otherwise n_sel[0] = n_iter.  ds: workspace of n doubles.
+ * perm: (n_iter,) int32, lambdas: (n_iter,) f64. */
+int rvgp_fps(const double* x, int n, int D, int n_iter, int start_idx, double spacing,
+             int use_spacing, const double* diam2, double* ds, int* perm, double* lambdas,
+             int* n_sel, void* stream);
+
+/* ---------------------------------------------------------------------
+ * a2  kNN graph  -- replaces geometry.manifold_graph(typ='knn')
+ *     (RVGP/geometry.py:100-123: sklearn kneighbors_graph(include_self=False)
+ *     + I -> networkx) and the symmetrisation of ptu_dijkstra.pyx:84-103.
+ * ------------------------------------------------------------------- */
+
+/* nbr (n, k) int32: the k nearest other points of every point, ordered by
+ * (squared distance, index).  Requires 0 < k < n, k <= 64, D <= 64. */
+int rvgp_knn(const double* x, int n, int D, int k, int* nbr, void* stream);
+
+/* Workspace bytes for rvgp_graph_symmetrize. */
+size_t rvgp_graph_symmetrize_ws_bytes(int n, int k);
+
+/* Sorted, de-duplicated CSR of max(A, A^T) with A = kNN connectivity + I:
+ * row i = {i} u nbr(i) u {j : i in nbr(j)}.  rowptr: (n+1) int32;
+ * cols: capacity n*(2k+1) int32.  nnz = rowptr[n]. */
+int rvgp_graph_symmetrize(const int* nbr, int n, int k, int* rowptr, int* cols, void* ws,
+                          size_t ws_bytes, void* stream);
+
+/* ---------------------------------------------------------------------
+ * a3  Geodesic neighbourhoods + tangent frames -- replaces
+ *     ptu_dijkstra.tangent_frames (RVGP/lib/ptu_dijkstra.pyx:33-125, 300-437).
+ * ------------------------------------------------------------------- */
+
+/* Workspace bytes for rvgp_geo_nbh with `batch` sources in flight and a
+ * node pool of `pool` slots per source (pool >= 1 + K * max row length). */
+size_t rvgp_geo_nbh_ws_bytes(int batch, int pool);
+
+/* Truncated Dijkstra driven by the Fibonacci heap of ptu_dijkstra.pyx:
+ * 440-690, one source per lane: nbh (n, K+1) int32 in pop order
+ * (pyx:361-394).  w: (nnz,) f64 edge weights (all 1.0 on the RVGP path).
+ * status[0] is set to RVGP_ST_SHORT_COMPONENT / RVGP_ST_POOL_OVERFLOW. */
+int rvgp_geo_nbh(const int* rowptr, const int* cols, const double* w, int n, int K, int batch,
+                 int pool, int* nbh, void* ws, size_t ws_bytes, int* status, void* stream);
+
+/* Centre each neighbourhood (pyx:397-404) and take its local PCA -- the left
+ * singular vectors and singular values of the D x (K+1) centred matrix
+ * (pyx:406-434) -- via an LDS-staged covariance and a Jacobi eigensolve,
+ * one wavefront per node.  tangents (n, D, ncols) f64 (first ncols left
+ * singular vectors, ncols <= min(D, K+1)); sigma (n, nsv) f64.  Any of the
+ * first n_check singular values below 1e-10 sets status[0] =
+ * RVGP_ST_RANK_DEFICIENT (pyx:426-428).  D <= 64, K+1 <= 64. */
+int rvgp_gauges(const double* x, int n, int D, const int* nbh, int Kp1, int ncols, int nsv,
+                int n_check, double* tangents, double* sigma, int* status, void* stream);
+
+/* ---------------------------------------------------------------------
+ * a5/a6/a7  Connections and Laplacians -- replaces ptu_dijkstra.connections
+ *     (pyx:128-293), geometry.compute_laplacian (geometry.py:55-63) and
+ *     geometry.compute_connection_laplacian (geometry.py:14-52).
+ * ------------------------------------------------------------------- */
+
+/* For every CSR entry e = (i, j): R_e = polar factor of T_i^T T_j (= U V^T
+ * of its SVD, pyx:259-293), T_i = tangents[i][:, :d] with row stride ldt;
+ * nnz = rowptr[n].
+ * Writes R (nnz, d, d) f64, and if non-NULL the connection-Laplacian blocks
+ * L_ij R_e (L_ii = degree, L_ij = -1) as f64 lc64 and f32 lc32, and the
+ * scalar Laplacian values as f64 l64 / f32 l32 (nnz,).  d <= 8. */
+int rvgp_connections(const double* tangents, int n, int D, int ldt, int d, const int* rowptr,
+                     const int* cols, int nnz, double* R, double* lc64, float* lc32, double* l64,
+                     float* l32, void* stream);
+
+/* ---------------------------------------------------------------------
+ * Block-sparse (BSR, b x b blocks, b in {1,2,3}) times dense, fused:
+ *   Y = alpha * (A X) + beta * X + gamma * Z ;  if acc: acc += delta * Y
+ * X, Z, Y, acc: (nb*b, r) with row strides ldx/ldz/ldy/ldacc.  Z and acc may
+ * be NULL.  Y must not alias X.  Used by the eigensolver (Chebyshev filter,
+ * Rayleigh-Ritz) and by the heat-kernel recurrence, in place of scipy sparse
+ * products inside eigsh (geometry.py:73) and dense expm (smoothing.py:15).
+ * ------------------------------------------------------------------- */
+int rvgp_bsr_spmm_f64(int nb, int b, const int* rowptr, const int* cols, const double* vals,
+                      const double* X, int ldx, int r, double alpha, double beta,
+                      const double* Z, int ldz, double gamma, double* Y, int ldy,
+                      double* acc, int ldacc, double delta, void* stream);
+int rvgp_bsr_spmm_f32(int nb, int b, const int* rowptr, const int* cols, const float* vals,
+                      const float* X, int ldx, int r, float alpha, float beta, const float* Z,
+                      int ldz, float gamma, float* Y, int ldy, float* acc, int ldacc,
+                      float delta, void* stream);
+
+/* ---------------------------------------------------------------------
+ * Dense linear algebra on the matrix cores.  Replaces the dense BLAS/LAPACK
+ * inside ARPACK's Rayleigh-Ritz (geometry.py:73), the GPflow Gram
+ * K = (X * S) X^T (kernels.py:55-61) and TensorFlow's Cholesky / triangular
+ * solves behind GPR.log_marginal_likelihood and predict_f (main.py:98-116).
+ * ------------------------------------------------------------------- */
+#define RVGP_GEMM_F64 0       /* f64 in, f64 MFMA, f64 out */
+#define RVGP_GEMM_F32 1       /* f32 in, f32 MFMA, f32 out */
+#define RVGP_GEMM_F32_ACC64 2 /* f32 in, widened, f64 MFMA, f64 out */
+#define RVGP_GEMM_BF16 3      /* f32 in, rounded to bf16, f32 MFMA accumulate, f32 out */
+
+/* C = alpha * op(A) * diag(kscale) * op(B) + beta * C, row-major operands,
+ * op = transpose when trans* != 0; kscale (K,) f64 or NULL.  split_k > 1
+ * splits the K range over that many workgroup slices whose partials are
+ * reduced in a fixed order (deterministic); it needs ws_bytes >=
+ * split_k * M * N * sizeof(out element). */
+int rvgp_gemm(int mode, int transA, int transB, int M, int N, int K, double alpha, const void* A,
+              int lda, const void* B, int ldb, const double* kscale, double beta, void* C, int ldc,
+              int split_k, void* ws, size_t ws_bytes, void* stream);
+
+/* In-place lower Cholesky A = L L^T of an n x n row-major matrix (f64 or
+ * f32 by `f64` flag); the strictly upper triangle is left unspecified.  A
+ * non-positive pivot sets status[0] = RVGP_ST_NOT_SPD (the condition that
+ * makes tf.linalg.cholesky fail inside GPflow). */
+int rvgp_potrf(int f64, int n, void* A, int lda, int* status, void* stream);
+
+/* Solve with the Cholesky factor in place on B (n x r):
+ *   trans == 0: L X = B ;  trans != 0: L^T X = B. */
+int rvgp_trsm(int f64, int trans, int n, int r, const void* L, int lda, void* B, int ldb,
+              void* stream);
+
+/* out[0] = sum_i log(L_ii) accumulated in f64 (half the log-determinant). */
+int rvgp_chol_logdet(int f64, int n, const void* L, int lda, double* out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RVGP_H */

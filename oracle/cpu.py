@@ -1,0 +1,472 @@
+"""FP64 CPU restatement of the RVGP connection-Laplacian GP path (the oracle).
+
+TEST INFRASTRUCTURE ONLY -- see oracle/__init__.py.
+
+Each function cites the reference lines it restates.  How each one is pinned
+(checked against something other than itself) is recorded here and exercised by
+tests/test_oracle.py:
+
+  fps                 geometry.py:126-162   pinned: same loop over sklearn
+                      ``pairwise_distances`` (``metric="sklearn"``) vs the
+                      direct-formula C loop; brute-force invariants.
+  knn / sym_csr       geometry.py:100-112   pinned: sklearn ``kneighbors_graph``
+                      + networkx ``adjacency_matrix`` on the same points.
+  geo_nbh             ptu_dijkstra.pyx:361-404  pinned: golden Sigma/tangents
+                      written by the reference's own compiled module
+                      (tests/golden/heap_*.npz).
+  tangent_frames      ptu_dijkstra.pyx:33-125, 406-434  pinned: golden vectors.
+  manifold_dimension  geometry.py:83-97     pinned: golden dims + closed form.
+  connections         ptu_dijkstra.pyx:128-293  pinned: golden R (COO) and
+                      ``scipy.linalg.polar``.
+  laplacian           geometry.py:55-63     pinned: ``networkx.laplacian_matrix``.
+  connection_laplacian geometry.py:14-52    pinned: restated scipy kron/multiply
+                      on the golden R; ring-holonomy closed form.
+  compute_spectrum    geometry.py:66-80     pinned: ``numpy.linalg.eigh`` and the
+                      ring-graph closed form 2-2cos(2 pi j/n).
+  smooth_vector_field dataclass.py:105-120, smoothing.py:8-64  pinned:
+                      ``scipy.linalg.expm`` vs eigen form; exp(-tL)1 = 1.
+  spectral_filter     kernels.py:41-53      pinned: closed form + finite diffs.
+  gpr_*               GPflow GPR (main.py:98-116)  pinned: textbook GPR
+                      (Rasmussen & Williams Alg. 2.1) vs a dense
+                      ``scipy.stats.multivariate_normal`` log-density, central
+                      finite differences; GPflow itself is absent, so the
+                      GPflow-specific optimiser trajectory is parity unpinned.
+  train_test_split    main.py:40-45         pinned: sklearn ``train_test_split``.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+import scipy.linalg
+import scipy.sparse as sp
+import scipy.sparse.linalg
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def build():
+    """Compile oracle/oracle.c into oracle/liboracle.so (gcc)."""
+    subprocess.check_call(["make", "-s", "-C", _HERE, "-B", "liboracle.so"])
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = ctypes.CDLL(path)
+        P = ctypes.c_void_p
+        L.oracle_diam2.restype = ctypes.c_double
+        L.oracle_diam2.argtypes = [P, ctypes.c_int, ctypes.c_int]
+        L.oracle_fps.restype = ctypes.c_int
+        L.oracle_fps.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                 ctypes.c_int, P, P]
+        L.oracle_knn.restype = ctypes.c_int
+        L.oracle_knn.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, P]
+        L.oracle_sym_csr.restype = ctypes.c_int
+        L.oracle_sym_csr.argtypes = [P, ctypes.c_int, ctypes.c_int, P, P]
+        L.oracle_geo_nbh.restype = ctypes.c_int
+        L.oracle_geo_nbh.argtypes = [P, ctypes.c_int, ctypes.c_int, P, P, P, ctypes.c_int, P, P]
+        _LIB = L
+    return _LIB
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+# --------------------------------------------------------------------------
+# a1  furthest point sampling  (geometry.py:126-162)
+# --------------------------------------------------------------------------
+def fps(x, N=None, spacing=0.1, start_idx=0, metric="direct"):
+    """Greedy farthest-point order.  ``metric="direct"`` runs the C loop with
+    the fixed direct distance formula; ``metric="sklearn"`` runs the same loop
+    over ``sklearn.metrics.pairwise_distances`` as the reference does."""
+    if spacing == 0.0:
+        return np.arange(len(x)), None
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    n, D = x.shape
+    if metric == "sklearn":
+        from sklearn.metrics import pairwise_distances
+        Dm = pairwise_distances(x)
+        nn = Dm.shape[0] if N is None else N
+        diam = Dm.max()
+        perm = np.zeros(nn, dtype=np.int32)
+        perm[0] = start_idx
+        lambdas = np.zeros(nn)
+        ds = Dm[start_idx, :]
+        for i in range(1, nn):
+            idx = np.argmax(ds)
+            perm[i] = idx
+            lambdas[i] = ds[idx]
+            ds = np.minimum(ds, Dm[idx, :])
+            if N is None and lambdas[i] / diam < spacing:
+                perm, lambdas = perm[:i], lambdas[:i]
+                break
+        return perm, lambdas
+    nn = n if N is None else int(N)
+    perm = np.zeros(nn, dtype=np.int32)
+    lambdas = np.zeros(nn)
+    cnt = lib().oracle_fps(_p(x), n, D, -1 if N is None else nn, float(spacing), int(start_idx),
+                           _p(perm), _p(lambdas))
+    if cnt < 0:
+        raise MemoryError("oracle_fps")
+    return perm[:cnt], lambdas[:cnt]
+
+
+def diameter(x):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    return float(np.sqrt(lib().oracle_diam2(_p(x), x.shape[0], x.shape[1])))
+
+
+# --------------------------------------------------------------------------
+# a2  kNN graph  (geometry.py:100-112)
+# --------------------------------------------------------------------------
+def knn(x, k):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    n, D = x.shape
+    nbr = np.empty((n, k), dtype=np.int32)
+    rc = lib().oracle_knn(_p(x), n, D, int(k), _p(nbr))
+    if rc != 0:
+        raise ValueError("oracle_knn: bad k")
+    return nbr
+
+
+def sym_csr(nbr):
+    nbr = np.ascontiguousarray(nbr, dtype=np.int32)
+    n, k = nbr.shape
+    rowptr = np.empty(n + 1, dtype=np.int32)
+    cols = np.empty(n * (2 * k + 1), dtype=np.int32)
+    nnz = lib().oracle_sym_csr(_p(nbr), n, k, _p(rowptr), _p(cols))
+    if nnz < 0:
+        raise MemoryError("oracle_sym_csr")
+    return rowptr, cols[:nnz].copy()
+
+
+def graph_csr_networkx(x, n_neighbors):
+    """The symmetrised CSR exactly as the reference obtains it: sklearn
+    kneighbors_graph + I (geometry.py:103-112) -> networkx -> adjacency matrix
+    -> max(A, A^T) (ptu_dijkstra.pyx:84-103)."""
+    import networkx as nx
+    from sklearn.neighbors import kneighbors_graph
+    A = kneighbors_graph(x, n_neighbors, mode="connectivity", metric="minkowski", p=2,
+                         metric_params=None, include_self=False, n_jobs=None)
+    A += sp.eye(A.shape[0])
+    G = nx.from_scipy_sparse_array(A)
+    C = sp.csr_matrix(nx.adjacency_matrix(G, weight="weight"), dtype=np.float64)
+    S = C.maximum(C.T.tocsr()).tocsr()
+    S.sort_indices()
+    return S.indptr.astype(np.int32), S.indices.astype(np.int32), S.data
+
+
+# --------------------------------------------------------------------------
+# a3  heap neighbourhoods + tangent frames  (ptu_dijkstra.pyx:33-125, 300-437)
+# --------------------------------------------------------------------------
+def geo_nbh(x, rowptr, cols, K, weights=None, centred=True):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    n, D = x.shape
+    w = np.ones(len(cols)) if weights is None else np.ascontiguousarray(weights, np.float64)
+    nbh = np.empty((n, K + 1), dtype=np.int32)
+    cm = np.empty((n, D, K + 1)) if centred else None
+    rc = lib().oracle_geo_nbh(_p(x), n, D, _p(np.ascontiguousarray(rowptr, np.int32)),
+                              _p(np.ascontiguousarray(cols, np.int32)), _p(w), int(K), _p(nbh),
+                              _p(cm) if centred else None)
+    if rc == -3:
+        raise MemoryError("Failed to allocate memory in geo_nbh")
+    if rc <= -1000:
+        raise ValueError(f"node {-rc - 1001} reaches fewer than K+1={K + 1} nodes")
+    return nbh, cm
+
+
+def tangent_frames(x, rowptr, cols, d, K, allow_rank_deficient_k=False):
+    """ptu_dijkstra.pyx:33-125.  Validation as pyx:68-82.  With
+    ``allow_rank_deficient_k`` the K<d check is relaxed to the build's
+    documented extension (min(D, K+1) singular values)."""
+    N, D = x.shape
+    K = int(K)
+    if K >= N:
+        raise ValueError("Geodesic neighborhood size must be less than the total number of samples")
+    if K < d and not allow_rank_deficient_k:
+        raise ValueError("Geodesic neighborhood size must be larger or equal to the embedding dimension")
+    if D < d:
+        raise ValueError("Embedding dimension must be less or equal to the ambient dimension of input data")
+    _, cm = geo_nbh(x, rowptr, cols, K)
+    U, S, _ = np.linalg.svd(cm, full_matrices=False)
+    d_eff = min(d, S.shape[1])
+    # pyx:426-428 checks the first d values.  In the extension, K+1 centred
+    # points span at most K directions, so only the first min(d, K) are checked.
+    n_check = min(d_eff, K) if allow_rank_deficient_k else d_eff
+    if np.any(S[:, :n_check] < 1e-10):
+        raise RuntimeError("Local tangent space approximation failed, at least one geodesic "
+                           "neighborhood does not span d-dimensional space")
+    return np.ascontiguousarray(U[:, :, :d_eff]), np.ascontiguousarray(S[:, :d_eff])
+
+
+def manifold_dimension(Sigma, frac_explained=0.9):
+    """geometry.py:83-97 (in place on a copy)."""
+    if frac_explained == 1.0:
+        return Sigma.shape[1]
+    S = np.array(Sigma, dtype=np.float64) ** 2
+    S /= S.sum(1, keepdims=True)
+    S = S.cumsum(1)
+    var_exp = S.mean(0) - S.std(0)
+    return int(np.where(var_exp >= frac_explained)[0][0] + 1)
+
+
+# --------------------------------------------------------------------------
+# a5  connections  (ptu_dijkstra.pyx:128-293)
+# --------------------------------------------------------------------------
+def connections(tangents, rowptr, cols):
+    """R_e = U V^T of SVD(T_i^T T_j) for every CSR entry (i, j), self loops
+    included, in row-major sorted order (pyx:259-293).  Returns (nnz, d, d)."""
+    n = tangents.shape[0]
+    rows = np.repeat(np.arange(n), np.diff(rowptr))
+    M = np.einsum("eki,ekj->eij", tangents[rows], tangents[cols])
+    U, _, Vt = np.linalg.svd(M)
+    return U @ Vt
+
+
+def connections_coo(R, rowptr, cols):
+    """The COO matrix that ptu_dijkstra.connections returns (pyx:194-214)."""
+    n = len(rowptr) - 1
+    d = R.shape[-1]
+    rows = np.repeat(np.arange(n), np.diff(rowptr))
+    p, q = np.meshgrid(np.arange(d), np.arange(d), indexing="ij")
+    r = (rows[:, None, None] * d + p[None]).ravel()
+    c = (cols[:, None, None] * d + q[None]).ravel()
+    return sp.coo_matrix((R.ravel(), (r, c)), shape=(n * d, n * d))
+
+
+# --------------------------------------------------------------------------
+# a6 / a7  Laplacians  (geometry.py:14-63)
+# --------------------------------------------------------------------------
+def laplacian(rowptr, cols):
+    """L = D - A of the simple symmetrised graph (self loops cancel)."""
+    n = len(rowptr) - 1
+    rows = np.repeat(np.arange(n), np.diff(rowptr))
+    A = sp.csr_matrix((np.ones(len(cols)), cols, rowptr), shape=(n, n))
+    deg = np.asarray(A.sum(1)).ravel()
+    return (sp.diags(deg) - A).tocsr()
+
+
+def connection_laplacian(rowptr, cols, R):
+    """kron(L, ones(d,d)) .multiply(R) (geometry.py:37-42) on the block
+    pattern; returns csr (nd x nd)."""
+    d = R.shape[-1]
+    L = laplacian(rowptr, cols)
+    Lk = sp.kron(L, np.ones((d, d)))
+    return sp.csr_matrix(Lk.multiply(connections_coo(R, rowptr, cols)))
+
+
+def lc_blocks(rowptr, cols, R):
+    """Block values of Lc in CSR order: L_ij R_ij (diag: deg_i R_ii)."""
+    n = len(rowptr) - 1
+    rows = np.repeat(np.arange(n), np.diff(rowptr))
+    deg = np.diff(rowptr) - 1
+    lv = np.where(rows == cols, deg[rows], -1.0)
+    return R * lv[:, None, None]
+
+
+# --------------------------------------------------------------------------
+# a8  spectrum  (geometry.py:66-80)
+# --------------------------------------------------------------------------
+def compute_spectrum(A, n_eigenpairs=None, dense=False):
+    """eigsh(k, which='SM'), ascending, eigenvectors scaled by sqrt(rows).
+    ``dense=True`` uses numpy.linalg.eigh (the library cross-check)."""
+    N = A.shape[0]
+    k = N if n_eigenpairs is None else min(int(n_eigenpairs), N)
+    if dense:
+        w, V = np.linalg.eigh(A.toarray() if sp.issparse(A) else A)
+        w, V = w[:k], V[:, :k]
+    else:
+        if k >= N - 1:
+            raise TypeError("Cannot use scipy.linalg.eigh for sparse A with k >= N - 1.")
+        w, V = scipy.sparse.linalg.eigsh(A, k=k, which="SM")
+        o = np.argsort(w)
+        w, V = w[o], V[:, o]
+    return w, V * np.sqrt(len(V))
+
+
+# --------------------------------------------------------------------------
+# a9  data object  (dataclass.py:22-89)
+# --------------------------------------------------------------------------
+class Data:
+    def __init__(self, vertices, vectors=None, n_neighbors=10, frac_geodesic_neighbours=1.5,
+                 explained_variance=0.8, n_eigenpairs=None, allow_rank_deficient_k=False,
+                 graph="direct", dense_eigh=False, spectra=True):
+        X = np.ascontiguousarray(vertices, dtype=np.float64)
+        if graph == "networkx":
+            rowptr, cols, _ = graph_csr_networkx(X, n_neighbors)
+        else:
+            rowptr, cols = sym_csr(knn(X, n_neighbors))
+        D = X.shape[1]
+        K = int(n_neighbors * frac_geodesic_neighbours)
+        gauges, Sigma = tangent_frames(X, rowptr, cols, D, K, allow_rank_deficient_k)
+        dim_man = manifold_dimension(Sigma, explained_variance)
+        gauges = np.ascontiguousarray(gauges[:, :, :dim_man])
+        R = connections(gauges, rowptr, cols)
+        self.vertices, self.n, self.dim_man = X, X.shape[0], dim_man
+        self.rowptr, self.cols, self.gauges, self.Sigma, self.Rblocks = rowptr, cols, gauges, Sigma, R
+        self.L = laplacian(rowptr, cols)
+        self.Lc = connection_laplacian(rowptr, cols, R)
+        self.vectors = vectors
+        if spectra:
+            self.evals_L, self.evecs_L = compute_spectrum(self.L, n_eigenpairs, dense_eigh)
+            evals_Lc, U = compute_spectrum(self.Lc, n_eigenpairs, dense_eigh)
+            k = U.shape[1]
+            self.evals_Lc = evals_Lc
+            self.U = U
+            self.evecs_Lc = np.einsum("bij,bjk->bik", gauges, U.reshape(-1, dim_man, k)).reshape(-1, k)
+
+    # a10  dataclass.py:91-103
+    def random_vector_field(self, seed=0):
+        np.random.seed(seed)
+        v = np.random.uniform(low=-0.5, high=0.5, size=(len(self.vertices), self.vertices.shape[1]))
+        v = project_to_manifold(v, self.gauges[..., :self.dim_man])
+        v /= np.linalg.norm(v, axis=1, keepdims=True)
+        self.vectors = v
+
+    # a11  dataclass.py:105-120
+    def smooth_vector_field(self, t=100, method="expm"):
+        c = express_in_local_frame(self.vectors, self.gauges)
+        c = vector_diffusion(c, t, self.Lc, self.L, method)
+        self.vectors = express_in_local_frame(c, self.gauges, reverse=True)
+
+
+def project_to_manifold(x, gauges):
+    """geometry.py:165-168."""
+    coeffs = np.einsum("bij,bi->bj", gauges, x)
+    return np.einsum("bj,bij->bi", coeffs, gauges)
+
+
+def express_in_local_frame(x, gauges, reverse=False):
+    """geometry.py:171-175."""
+    if reverse:
+        return np.einsum("bji,bi->bj", gauges, x)
+    return np.einsum("bij,bi->bj", gauges, x)
+
+
+def heat(A, t, x, method="expm"):
+    """exp(-tA) x.  ``expm``: scipy.linalg.expm of the dense matrix, as
+    smoothing.py:13-15; ``eigh``: V exp(-t w) V^T x, the same operator."""
+    Ad = A.toarray() if sp.issparse(A) else np.asarray(A)
+    if method == "expm":
+        return scipy.linalg.expm(-t * Ad) @ x
+    w, V = np.linalg.eigh(Ad)
+    return V @ (np.exp(-t * w)[:, None] * (V.T @ x))
+
+
+def vector_diffusion(x, t, Lc, L, method="expm"):
+    """smoothing.py:37-64 with normalise=True."""
+    n, d = x.shape
+    out = heat(Lc, t, x.reshape(n * d, -1), method).reshape(x.shape)
+    x_abs = np.linalg.norm(x, axis=-1, keepdims=True)
+    out_abs = heat(L, t, x_abs, method)
+    ind = heat(L, t, np.ones([n, 1]), method)
+    return out * out_abs / (ind * np.linalg.norm(out, axis=-1, keepdims=True))
+
+
+# --------------------------------------------------------------------------
+# a12  spectral filter  (kernels.py:41-53)
+# --------------------------------------------------------------------------
+def spectral_filter(evals, nu, kappa, sigma_f, n_rows, typ="matern"):
+    """S and dS/d(nu, kappa, sigma_f) including the normalisation
+    S = sigma_f * n_rows * r / sum(r)."""
+    lam = np.asarray(evals, dtype=np.float64)
+    if typ == "matern":
+        base = lam + 2.0 * nu / kappa ** 2
+        r = base ** (-nu)
+        dlog_nu = -np.log(base) - nu * (2.0 / kappa ** 2) / base
+        dlog_kappa = 4.0 * nu ** 2 / (kappa ** 3 * base)
+    else:
+        r = np.exp(-0.5 * lam * kappa ** 2)
+        dlog_nu = np.zeros_like(lam)
+        dlog_kappa = -lam * kappa
+    sr = r.sum()
+    S = sigma_f * n_rows * r / sr
+    w = r / sr
+    dS_nu = S * (dlog_nu - (w * dlog_nu).sum())
+    dS_kappa = S * (dlog_kappa - (w * dlog_kappa).sum())
+    dS_sigma = S / sigma_f
+    return S, np.stack([dS_nu, dS_kappa, dS_sigma])
+
+
+# --------------------------------------------------------------------------
+# a14 / a16  GP regression, textbook (Rasmussen & Williams Alg. 2.1)
+# --------------------------------------------------------------------------
+def gpr_nlml(Phi, y, S, noise):
+    """-log N(y | 0, Phi diag(S) Phi^T + noise I), function space."""
+    K = (Phi * S) @ Phi.T
+    M = K.shape[0]
+    L = np.linalg.cholesky(K + noise * np.eye(M))
+    a = scipy.linalg.solve_triangular(L, y, lower=True)
+    nlml = 0.5 * float((a * a).sum()) + float(np.log(np.diag(L)).sum()) + 0.5 * M * np.log(2 * np.pi)
+    return nlml
+
+
+def gpr_grad(Phi, y, S, noise):
+    """(dNLML/dS (k,), dNLML/dnoise) from 1/2 tr((Ky^-1 - a a^T) dKy)."""
+    K = (Phi * S) @ Phi.T
+    M = K.shape[0]
+    Ky = K + noise * np.eye(M)
+    Kinv = np.linalg.inv(Ky)
+    a = Kinv @ y.ravel()
+    PtKP = np.einsum("mj,mn,nj->j", Phi, Kinv, Phi)
+    Pta = Phi.T @ a
+    gS = 0.5 * (PtKP - Pta ** 2)
+    gn = 0.5 * (np.trace(Kinv) - a @ a)
+    return gS, gn
+
+
+def gpr_predict(Phi, y, S, noise, Phi_star):
+    """GPflow GPR.predict_f(full_cov=False): mean and marginal variance."""
+    K = (Phi * S) @ Phi.T
+    M = K.shape[0]
+    L = np.linalg.cholesky(K + noise * np.eye(M))
+    Kmn = (Phi * S) @ Phi_star.T
+    A = scipy.linalg.solve_triangular(L, Kmn, lower=True)
+    var = np.einsum("ij,j,ij->i", Phi_star, S, Phi_star) - (A * A).sum(0)
+    a = scipy.linalg.solve_triangular(L, y, lower=True)
+    mean = A.T @ a
+    return mean, var[:, None]
+
+
+# --------------------------------------------------------------------------
+# a13 / a15  fit driver  (main.py:11-95)
+# --------------------------------------------------------------------------
+def train_test_split_indices(n, test_size=0.2, seed=0):
+    """sklearn ShuffleSplit as used by train_test_split(random_state=seed):
+    test = perm[:ceil(test_size n)], train = perm[n_test:]."""
+    n_test = int(np.ceil(test_size * n))
+    perm = np.random.RandomState(seed).permutation(n)
+    return perm[n_test:], perm[:n_test]
+
+
+def softplus(u):
+    return np.logaddexp(0.0, u)
+
+
+def inv_softplus(x):
+    return x + np.log(-np.expm1(-x))
+
+
+def fit_lbfgs(Phi, y, evals, n_rows, theta0=(1.5, 5.0, 1.0, 1.0), lower=(0.0, 0.0, 0.0, 1e-6),
+              maxiter=1000):
+    """L-BFGS-B on softplus-unconstrained (nu, kappa, sigma_f, noise)."""
+    import scipy.optimize
+    lower = np.asarray(lower, dtype=np.float64)
+    u0 = inv_softplus(np.asarray(theta0, dtype=np.float64) - lower)
+
+    def f(u):
+        th = softplus(u) + lower
+        S, dS = spectral_filter(evals, th[0], th[1], th[2], n_rows)
+        val = gpr_nlml(Phi, y, S, th[3])
+        gS, gn = gpr_grad(Phi, y, S, th[3])
+        g = np.array([dS[0] @ gS, dS[1] @ gS, dS[2] @ gS, gn])
+        return val, g * (1.0 / (1.0 + np.exp(-u)))
+
+    res = scipy.optimize.minimize(f, u0, jac=True, method="L-BFGS-B", options={"maxiter": maxiter})
+    return softplus(res.x) + lower, res

@@ -1,0 +1,188 @@
+// a3 (part 2)  Tangent frames: centring (RVGP/lib/ptu_dijkstra.pyx:397-404)
+// and the SVD of the D x (K+1) neighbourhood matrix (pyx:406-434).
+//
+// One wavefront per node.  The neighbourhood is gathered into LDS (lanes
+// read consecutive coordinates of one point, so each point row is one
+// coalesced segment), centred with the mean summed in pop order exactly as
+// the reference does, and reduced to the smaller of its two symmetric Gram
+// matrices (D x D covariance when D <= K+1, else (K+1) x (K+1)).  A cyclic
+// Jacobi eigensolve with the round-robin pair ordering rotates m/2 disjoint
+// pairs per step, the lanes splitting the row / column / vector updates.
+// Singular values are the square roots of the eigenvalues; the left singular
+// vectors are the eigenvectors (covariance case) or G v / s (Gram case).
+// Singular-vector signs are free, as they are for LAPACK's dgesvd.
+#include "common.h"
+
+namespace rvgp {
+
+__global__ __launch_bounds__(64) void gauges_kernel(const double* __restrict__ x, int n, int D,
+                                                    const int* __restrict__ nbh, int Kp1,
+                                                    int ncols, int nsv, int n_check,
+                                                    double* __restrict__ tangents,
+                                                    double* __restrict__ sigma,
+                                                    int* __restrict__ status) {
+    extern __shared__ double smem[];
+    const int i = blockIdx.x;
+    const int lane = threadIdx.x;
+    const bool cov = D <= Kp1;
+    const int m = cov ? D : Kp1;
+    const int m2 = m + (m & 1);
+    double* G = smem;                  // D x Kp1
+    double* A = G + D * Kp1;           // m2 x m2
+    double* V = A + m2 * m2;           // m2 x m2
+    double* cs = V + m2 * m2;          // m2/2 cosines
+    double* sn = cs + m2 / 2;          // m2/2 sines
+    int* pp = reinterpret_cast<int*>(sn + m2 / 2);  // m2/2
+    int* qq = pp + m2 / 2;                          // m2/2
+    int* idx = qq + m2 / 2;                         // Kp1
+    int* order = idx + Kp1;                         // m
+
+    for (int q = lane; q < Kp1; q += 64) idx[q] = nbh[(size_t)i * Kp1 + q];
+    __syncthreads();
+    for (int e = lane; e < D * Kp1; e += 64) {
+        int q = e / D, p = e - q * D;
+        G[p * Kp1 + q] = x[(size_t)idx[q] * D + p];
+    }
+    __syncthreads();
+    for (int p = lane; p < D; p += 64) {  // pyx:397-404
+        double mean = 0.0;
+        for (int q = 0; q < Kp1; ++q) mean += G[p * Kp1 + q];
+        mean = mean / Kp1;
+        for (int q = 0; q < Kp1; ++q) G[p * Kp1 + q] -= mean;
+    }
+    __syncthreads();
+    for (int e = lane; e < m2 * m2; e += 64) {
+        int a = e / m2, b = e - a * m2;
+        double s = 0.0;
+        if (a < m && b < m) {
+            if (cov) {
+                for (int q = 0; q < Kp1; ++q) s = __fma_rn(G[a * Kp1 + q], G[b * Kp1 + q], s);
+            } else {
+                for (int p = 0; p < D; ++p) s = __fma_rn(G[p * Kp1 + a], G[p * Kp1 + b], s);
+            }
+        }
+        A[e] = s;
+        V[e] = (a == b) ? 1.0 : 0.0;
+    }
+    __syncthreads();
+
+    const int half = m2 / 2;
+    for (int sweep = 0; sweep < 40; ++sweep) {
+        double off = 0.0, dia = 0.0;
+        for (int e = lane; e < m2 * m2; e += 64) {
+            int a = e / m2, b = e - a * m2;
+            double v = A[e] * A[e];
+            if (a == b) dia += v; else off += v;
+        }
+        off = wave_sum(off);
+        dia = wave_sum(dia);
+        if (!(off > 1e-32 * dia)) break;  // also stops on an all-zero matrix
+        for (int step = 0; step < m2 - 1; ++step) {
+            if (lane < half) {
+                // round robin: position 0 fixed, positions 1..m2-1 rotate
+                int a = (lane == 0) ? 0 : 1 + (lane - 1 + step) % (m2 - 1);
+                int bpos = m2 - 1 - lane;
+                int b = 1 + (bpos - 1 + step) % (m2 - 1);
+                int p = a < b ? a : b, q = a < b ? b : a;
+                double apq = A[p * m2 + q];
+                double c = 1.0, s = 0.0;
+                if (apq != 0.0) {
+                    double tau = (A[q * m2 + q] - A[p * m2 + p]) / (2.0 * apq);
+                    double t;
+                    if (fabs(tau) > 1e150) t = 0.5 / tau;
+                    else t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
+                    c = 1.0 / sqrt(1.0 + t * t);
+                    s = t * c;
+                }
+                pp[lane] = p;
+                qq[lane] = q;
+                cs[lane] = c;
+                sn[lane] = s;
+            }
+            __syncthreads();
+            for (int e = lane; e < half * m2; e += 64) {  // A <- A J, V <- V J
+                int k = e / m2, r = e - k * m2;
+                int p = pp[k], q = qq[k];
+                double c = cs[k], s = sn[k];
+                double arp = A[r * m2 + p], arq = A[r * m2 + q];
+                A[r * m2 + p] = c * arp - s * arq;
+                A[r * m2 + q] = s * arp + c * arq;
+                double vrp = V[r * m2 + p], vrq = V[r * m2 + q];
+                V[r * m2 + p] = c * vrp - s * vrq;
+                V[r * m2 + q] = s * vrp + c * vrq;
+            }
+            __syncthreads();
+            for (int e = lane; e < half * m2; e += 64) {  // A <- J^T A
+                int k = e / m2, r = e - k * m2;
+                int p = pp[k], q = qq[k];
+                double c = cs[k], s = sn[k];
+                double apr = A[p * m2 + r], aqr = A[q * m2 + r];
+                A[p * m2 + r] = c * apr - s * aqr;
+                A[q * m2 + r] = s * apr + c * aqr;
+            }
+            __syncthreads();
+        }
+    }
+    // eigenvalues descending (ties: lower index first)
+    for (int a = lane; a < m; a += 64) {
+        double la = A[a * m2 + a];
+        int rank = 0;
+        for (int b = 0; b < m; ++b) {
+            double lb = A[b * m2 + b];
+            rank += (lb > la) || (lb == la && b < a);
+        }
+        order[rank] = a;
+    }
+    __syncthreads();
+    int bad = 0;
+    for (int r = lane; r < m; r += 64) {
+        double lam = A[order[r] * m2 + order[r]];
+        double s = sqrt(lam > 0.0 ? lam : 0.0);
+        if (r < nsv) sigma[(size_t)i * nsv + r] = s;
+        if (r < n_check && s < 1e-10) bad = 1;
+    }
+    for (int e = lane; e < D * ncols; e += 64) {
+        int p = e / ncols, r = e - p * ncols;
+        int c = order[r];
+        double u;
+        if (cov) {
+            u = V[p * m2 + c];
+        } else {
+            double lam = A[c * m2 + c];
+            double s = sqrt(lam > 0.0 ? lam : 0.0);
+            double acc = 0.0;
+            for (int q = 0; q < Kp1; ++q) acc = __fma_rn(G[p * Kp1 + q], V[q * m2 + c], acc);
+            u = s > 0.0 ? acc / s : 0.0;
+        }
+        tangents[(size_t)i * D * ncols + e] = u;
+    }
+    if (__any(bad) && lane == 0) atomicMax(status, RVGP_ST_RANK_DEFICIENT);
+}
+
+static size_t gauges_smem(int D, int Kp1) {
+    int m = D <= Kp1 ? D : Kp1;
+    int m2 = m + (m & 1);
+    size_t dbl = (size_t)D * Kp1 + 2 * (size_t)m2 * m2 + (size_t)m2;
+    size_t ints = 2 * (size_t)(m2 / 2) + Kp1 + m;
+    return dbl * sizeof(double) + ints * sizeof(int);
+}
+
+}  // namespace rvgp
+
+using namespace rvgp;
+
+extern "C" int rvgp_gauges(const double* x, int n, int D, const int* nbh, int Kp1, int ncols,
+                           int nsv, int n_check, double* tangents, double* sigma, int* status,
+                           void* stream) {
+    RVGP_CHECK_ARG(n >= 1 && D >= 1 && D <= 64 && Kp1 >= 2 && Kp1 <= 64,
+                   "rvgp_gauges: need D <= 64 and 2 <= K+1 <= 64");
+    const int m = D <= Kp1 ? D : Kp1;
+    RVGP_CHECK_ARG(ncols >= 0 && ncols <= m && nsv >= 0 && nsv <= m && n_check <= m,
+                   "rvgp_gauges: ncols/nsv/n_check must not exceed min(D, K+1)");
+    const size_t sm = gauges_smem(D, Kp1);
+    RVGP_CHECK_ARG(sm <= 65536, "rvgp_gauges: D=%d, K+1=%d needs %zu bytes of LDS (> 64 KiB)", D, Kp1, sm);
+    hipLaunchKernelGGL(gauges_kernel, dim3(n), dim3(64), sm, as_stream(stream), x, n, D, nbh, Kp1,
+                       ncols, nsv, n_check, tangents, sigma, status);
+    RVGP_LAUNCH_CHECK();
+    return RVGP_OK;
+}

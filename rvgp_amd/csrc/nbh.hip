@@ -1,0 +1,279 @@
+// a3 (part 1)  Geodesic neighbourhoods: the truncated Dijkstra of
+// RVGP/lib/ptu_dijkstra.pyx:361-394 driven by the Fibonacci heap of
+// pyx:440-690.
+//
+// Which K+1 nodes form a neighbourhood is decided by the heap's pop order
+// among equal hop distances, so the heap is emulated operation for
+// operation: the same sibling-list splices, the same linking order, the same
+// choice of the new minimum.  One lane owns one source.  Instead of the
+// reference's N heap nodes per source (re-initialised N times, an O(N^2)
+// total), a lane keeps a private pool of only the nodes its search touches,
+// found through a small open-addressing hash table keyed by node id; node
+// identity (the reference compares pointers) becomes pool-slot identity.
+// Lanes stride over sources, so `batch` lanes handle any n.
+#include "common.h"
+
+namespace rvgp {
+
+enum : int { kScanned = 0, kNotInHeap = 1, kInHeap = 2 };
+constexpr int kMaxRank = 32;
+
+struct HNode {
+    double val;
+    int gidx, rank, state, parent, left, right, children, hpos;
+};
+
+struct Heap {
+    HNode* N;
+    int* roots;
+    int min_node;
+};
+
+__device__ __forceinline__ int rightmost_sibling(HNode* N, int a) {
+    while (N[a].right != -1) a = N[a].right;
+    return a;
+}
+
+__device__ __forceinline__ int leftmost_sibling(HNode* N, int a) {
+    while (N[a].left != -1) a = N[a].left;
+    return a;
+}
+
+__device__ __forceinline__ void add_sibling(HNode* N, int a, int s) {  // pyx:526-536
+    int t = rightmost_sibling(N, a);
+    N[t].right = s;
+    N[s].left = t;
+    N[s].right = -1;
+    N[s].parent = N[a].parent;
+    if (N[s].parent != -1) N[N[s].parent].rank += 1;
+}
+
+__device__ __forceinline__ void add_child(HNode* N, int a, int c) {  // pyx:507-520
+    N[c].parent = a;
+    if (N[a].children != -1) {
+        add_sibling(N, N[a].children, c);
+    } else {
+        N[a].children = c;
+        N[c].right = -1;
+        N[c].left = -1;
+        N[a].rank = 1;
+    }
+}
+
+__device__ __forceinline__ void node_remove(HNode* N, int a) {  // pyx:542-560
+    int p = N[a].parent;
+    if (p != -1) {
+        N[p].rank -= 1;
+        if (N[a].left != -1) N[p].children = N[a].left;
+        else if (N[a].right != -1) N[p].children = N[a].right;
+        else N[p].children = -1;
+    }
+    if (N[a].left != -1) N[N[a].left].right = N[a].right;
+    if (N[a].right != -1) N[N[a].right].left = N[a].left;
+    N[a].left = -1;
+    N[a].right = -1;
+    N[a].parent = -1;
+}
+
+__device__ __forceinline__ void insert_node(Heap& h, int a) {  // pyx:579-589
+    if (h.min_node != -1) {
+        add_sibling(h.N, h.min_node, a);
+        if (h.N[a].val < h.N[h.min_node].val) h.min_node = a;
+    } else {
+        h.min_node = a;
+    }
+}
+
+__device__ __forceinline__ void decrease_val(Heap& h, int a, double v) {  // pyx:595-608
+    HNode* N = h.N;
+    N[a].val = v;
+    if (N[a].parent != -1 && N[N[a].parent].val >= v) {
+        node_remove(N, a);
+        insert_node(h, a);
+    } else if (N[h.min_node].val > N[a].val) {
+        h.min_node = a;
+    }
+}
+
+// pyx:614-636 (tail recursion as a loop).  Returns false on rank overflow.
+__device__ __forceinline__ bool heap_link(Heap& h, int a) {
+    HNode* N = h.N;
+    for (;;) {
+        int r = N[a].rank;
+        if (r >= kMaxRank) return false;
+        if (h.roots[r] == -1) {
+            h.roots[r] = a;
+            return true;
+        }
+        int ln = h.roots[r];
+        h.roots[r] = -1;
+        if (N[a].val < N[ln].val || a == h.min_node) {
+            node_remove(N, ln);
+            add_child(N, a, ln);
+        } else {
+            node_remove(N, a);
+            add_child(N, ln, a);
+            a = ln;
+        }
+    }
+}
+
+// pyx:642-690.  The reference clears roots_by_rank[0..100); ranks here are
+// bounded by kMaxRank (checked), so clearing [0, kMaxRank) is equivalent.
+__device__ __forceinline__ int remove_min(Heap& h, bool& ok) {
+    HNode* N = h.N;
+    int m = h.min_node, t, tr;
+    if (N[m].children != -1) {
+        t = leftmost_sibling(N, N[m].children);
+        while (t != -1) {
+            tr = N[t].right;
+            node_remove(N, t);
+            add_sibling(N, m, t);
+            t = tr;
+        }
+        N[m].children = -1;
+    }
+    t = leftmost_sibling(N, m);
+    if (t == m) {
+        if (N[m].right != -1) {
+            t = N[m].right;
+        } else {
+            h.min_node = -1;
+            return m;
+        }
+    }
+    node_remove(N, m);
+    h.min_node = t;
+    for (int i = 0; i < kMaxRank; ++i) h.roots[i] = -1;
+    while (t != -1) {
+        if (N[t].val < N[h.min_node].val) h.min_node = t;
+        tr = N[t].right;
+        if (!heap_link(h, t)) { ok = false; return m; }
+        t = tr;
+    }
+    return m;
+}
+
+__device__ __forceinline__ unsigned hash_of(int g, unsigned mask) {
+    return ((unsigned)g * 2654435761u) & mask;
+}
+
+__global__ __launch_bounds__(256) void geo_nbh_kernel(const int* __restrict__ rowptr,
+                                                      const int* __restrict__ cols,
+                                                      const double* __restrict__ w, int n, int K,
+                                                      int batch, int pool, int hsize,
+                                                      size_t lane_bytes, int* __restrict__ nbh,
+                                                      char* __restrict__ ws,
+                                                      int* __restrict__ status) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= batch || t >= n) return;
+    char* base = ws + (size_t)t * lane_bytes;
+    HNode* N = reinterpret_cast<HNode*>(base);
+    int* keys = reinterpret_cast<int*>(N + pool);
+    int* slots = keys + hsize;
+    int* roots = slots + hsize;
+    const unsigned mask = (unsigned)hsize - 1u;
+    for (int q = 0; q < hsize; ++q) keys[q] = -1;
+    const int Kp1 = K + 1;
+
+    for (int src = t; src < n; src += batch) {
+        int used = 0;
+        bool ok = true;
+        auto new_node = [&](int g, unsigned hp) -> int {
+            int s = used++;
+            HNode& nd = N[s];
+            nd.val = 0.0;
+            nd.gidx = g;
+            nd.rank = 0;
+            nd.state = kNotInHeap;
+            nd.parent = nd.left = nd.right = nd.children = -1;
+            nd.hpos = (int)hp;
+            keys[hp] = g;
+            slots[hp] = s;
+            return s;
+        };
+        Heap h{N, roots, -1};
+        {
+            unsigned hp = hash_of(src, mask);
+            while (keys[hp] != -1) hp = (hp + 1) & mask;
+            insert_node(h, new_node(src, hp));
+        }
+        int scanned = 0;
+        int* out = nbh + (size_t)src * Kp1;
+        while (h.min_node != -1 && scanned <= K) {
+            int v = remove_min(h, ok);
+            if (!ok) break;
+            N[v].state = kScanned;
+            out[scanned++] = N[v].gidx;
+            if (scanned <= K) {
+                const int j = N[v].gidx;
+                const double vv = N[v].val;
+                for (int e = rowptr[j]; e < rowptr[j + 1]; ++e) {
+                    const int c = cols[e];
+                    unsigned hp = hash_of(c, mask);
+                    int sc = -1;
+                    while (keys[hp] != -1) {
+                        if (keys[hp] == c) { sc = slots[hp]; break; }
+                        hp = (hp + 1) & mask;
+                    }
+                    if (sc != -1 && N[sc].state == kScanned) continue;
+                    const double nv = vv + w[e];
+                    if (sc == -1 || N[sc].state == kNotInHeap) {
+                        if (sc == -1) {
+                            if (used >= pool) { ok = false; break; }
+                            sc = new_node(c, hp);
+                        }
+                        N[sc].state = kInHeap;
+                        N[sc].val = nv;
+                        insert_node(h, sc);
+                    } else if (N[sc].val > nv) {
+                        decrease_val(h, sc, nv);
+                    }
+                }
+                if (!ok) break;
+            }
+        }
+        for (int s = 0; s < used; ++s) keys[N[s].hpos] = -1;
+        if (!ok) {
+            atomicMax(status, RVGP_ST_POOL_OVERFLOW);
+            return;
+        }
+        if (scanned < Kp1) atomicMax(status, RVGP_ST_SHORT_COMPONENT);
+    }
+}
+
+static int next_pow2(int v) {
+    int p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+static size_t lane_bytes_for(int pool) {
+    const int hsize = next_pow2(2 * pool);
+    size_t b = sizeof(HNode) * (size_t)pool + sizeof(int) * (2 * (size_t)hsize + kMaxRank);
+    return (b + 255) & ~(size_t)255;
+}
+
+}  // namespace rvgp
+
+using namespace rvgp;
+
+extern "C" size_t rvgp_geo_nbh_ws_bytes(int batch, int pool) {
+    return lane_bytes_for(pool) * (size_t)batch;
+}
+
+extern "C" int rvgp_geo_nbh(const int* rowptr, const int* cols, const double* w, int n, int K,
+                            int batch, int pool, int* nbh, void* ws, size_t ws_bytes, int* status,
+                            void* stream) {
+    RVGP_CHECK_ARG(n >= 1 && K >= 0, "rvgp_geo_nbh: need n >= 1, K >= 0");
+    RVGP_CHECK_ARG(K < n, "Geodesic neighborhood size must be less than the total number of samples");
+    RVGP_CHECK_ARG(batch >= 1 && pool >= K + 1, "rvgp_geo_nbh: bad batch/pool");
+    RVGP_CHECK_ARG(ws_bytes >= rvgp_geo_nbh_ws_bytes(batch, pool), "rvgp_geo_nbh: workspace too small");
+    const int hsize = next_pow2(2 * pool);
+    const int lanes = batch < n ? batch : n;
+    hipLaunchKernelGGL(geo_nbh_kernel, dim3(cdiv(lanes, 256)), dim3(256), 0, as_stream(stream),
+                       rowptr, cols, w, n, K, lanes, pool, hsize, lane_bytes_for(pool), nbh,
+                       static_cast<char*>(ws), status);
+    RVGP_LAUNCH_CHECK();
+    return RVGP_OK;
+}

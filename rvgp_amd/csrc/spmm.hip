@@ -1,0 +1,200 @@
+// Block-sparse (BSR) times dense with a fused affine epilogue:
+//     Y = alpha (A X) + beta X + gamma Z ;   acc += delta Y   (optional)
+// This one kernel family carries the eigensolver (Chebyshev filter steps
+// Y_{k+1} = (2/e)(A - c) Y_k - Y_{k-1} and Rayleigh-Ritz products) and the
+// heat-kernel recurrence exp(-tA) x (the Chebyshev sum accumulates into
+// `acc`), replacing the sparse products inside ARPACK (RVGP/geometry.py:73)
+// and the dense scipy expm (RVGP/smoothing.py:13-15).
+//
+// Layout: block rows i = 0..nb-1, blocks of b x b values row-major, stored in
+// CSR order (rowptr / cols over block indices).  Dense operands are
+// (nb*b, r) row-major with explicit row strides.
+//
+// Two mappings:
+//  * narrow (r <= 4): LPR lanes share one block row and split its blocks
+//    (nnz-parallel), reducing with cross-lane shuffles -- consecutive lanes
+//    read consecutive blocks, so block values stream coalesced.
+//  * wide (r > 4): one wavefront per (block row, 64*CPL-column chunk); the
+//    block row index is wave-uniform, so block values and column indices
+//    are scalar loads, and every X row segment is a 64-lane coalesced load.
+//    Column chunks form the slow grid dimension so that all block rows of
+//    one chunk run together and the chunk of X they gather stays in cache.
+#include "common.h"
+
+namespace rvgp {
+
+template <typename T, int B, int LPR, int RMAX>
+__global__ __launch_bounds__(256) void spmm_narrow(
+    int nb, const int* __restrict__ rowptr, const int* __restrict__ cols,
+    const T* __restrict__ vals, const T* __restrict__ X, int ldx, int r, T alpha, T beta,
+    const T* __restrict__ Z, int ldz, T gamma, T* __restrict__ Y, int ldy, T* __restrict__ acc,
+    int ldacc, T delta) {
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int row = gid / LPR;
+    const int l = gid - row * LPR;
+    const bool live = row < nb;
+    T y[B][RMAX];
+#pragma unroll
+    for (int a = 0; a < B; ++a)
+#pragma unroll
+        for (int c = 0; c < RMAX; ++c) y[a][c] = T(0);
+    if (live) {
+        const int e1 = rowptr[row + 1];
+        for (int e = rowptr[row] + l; e < e1; e += LPR) {
+            const int j = cols[e];
+            const T* blk = vals + (size_t)e * B * B;
+            T v[B][B];
+#pragma unroll
+            for (int a = 0; a < B; ++a)
+#pragma unroll
+                for (int bb = 0; bb < B; ++bb) v[a][bb] = blk[a * B + bb];
+#pragma unroll
+            for (int c = 0; c < RMAX; ++c) {
+                if (c < r) {
+                    T xb[B];
+#pragma unroll
+                    for (int bb = 0; bb < B; ++bb) xb[bb] = X[(size_t)(j * B + bb) * ldx + c];
+#pragma unroll
+                    for (int a = 0; a < B; ++a)
+#pragma unroll
+                        for (int bb = 0; bb < B; ++bb) y[a][c] += v[a][bb] * xb[bb];
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int o = LPR / 2; o > 0; o >>= 1)
+#pragma unroll
+        for (int a = 0; a < B; ++a)
+#pragma unroll
+            for (int c = 0; c < RMAX; ++c) y[a][c] += __shfl_xor(y[a][c], o, LPR);
+    if (!live) return;
+#pragma unroll
+    for (int a = 0; a < B; ++a)
+#pragma unroll
+        for (int c = 0; c < RMAX; ++c) {
+            if (c < r && ((a * RMAX + c) % LPR) == l) {
+                const size_t rr = (size_t)(row * B + a);
+                T out = alpha * y[a][c];
+                if (beta != T(0)) out += beta * X[rr * ldx + c];
+                if (Z != nullptr && gamma != T(0)) out += gamma * Z[rr * ldz + c];
+                Y[rr * ldy + c] = out;
+                if (acc != nullptr) acc[rr * ldacc + c] += delta * out;
+            }
+        }
+}
+
+template <typename T, int B, int CPL>
+__global__ __launch_bounds__(256) void spmm_wide(
+    int nb, const int* __restrict__ rowptr, const int* __restrict__ cols,
+    const T* __restrict__ vals, const T* __restrict__ X, int ldx, int r, T alpha, T beta,
+    const T* __restrict__ Z, int ldz, T gamma, T* __restrict__ Y, int ldy, T* __restrict__ acc,
+    int ldacc, T delta) {
+    constexpr int WPB = 4;
+    const int lane = threadIdx.x & 63;
+    const int row = __builtin_amdgcn_readfirstlane(blockIdx.x * WPB + (threadIdx.x >> 6));
+    if (row >= nb) return;
+    const int c0 = blockIdx.y * (64 * CPL) + lane;
+    T y[B][CPL];
+#pragma unroll
+    for (int a = 0; a < B; ++a)
+#pragma unroll
+        for (int t = 0; t < CPL; ++t) y[a][t] = T(0);
+    const int e0 = rowptr[row], e1 = rowptr[row + 1];
+    for (int e = e0; e < e1; ++e) {
+        const int j = cols[e];
+        const T* blk = vals + (size_t)e * B * B;
+        T v[B][B];
+#pragma unroll
+        for (int a = 0; a < B; ++a)
+#pragma unroll
+            for (int bb = 0; bb < B; ++bb) v[a][bb] = blk[a * B + bb];
+        const T* xr = X + (size_t)j * B * ldx;
+#pragma unroll
+        for (int t = 0; t < CPL; ++t) {
+            const int c = c0 + 64 * t;
+            if (c < r) {
+                T xb[B];
+#pragma unroll
+                for (int bb = 0; bb < B; ++bb) xb[bb] = xr[(size_t)bb * ldx + c];
+#pragma unroll
+                for (int a = 0; a < B; ++a)
+#pragma unroll
+                    for (int bb = 0; bb < B; ++bb) y[a][t] += v[a][bb] * xb[bb];
+            }
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < B; ++a) {
+        const size_t rr = (size_t)(row * B + a);
+#pragma unroll
+        for (int t = 0; t < CPL; ++t) {
+            const int c = c0 + 64 * t;
+            if (c < r) {
+                T out = alpha * y[a][t];
+                if (beta != T(0)) out += beta * X[rr * ldx + c];
+                if (Z != nullptr && gamma != T(0)) out += gamma * Z[rr * ldz + c];
+                Y[rr * ldy + c] = out;
+                if (acc != nullptr) acc[rr * ldacc + c] += delta * out;
+            }
+        }
+    }
+}
+
+template <typename T>
+static int spmm_dispatch(int nb, int b, const int* rowptr, const int* cols, const T* vals,
+                         const T* X, int ldx, int r, T alpha, T beta, const T* Z, int ldz,
+                         T gamma, T* Y, int ldy, T* acc, int ldacc, T delta, void* stream) {
+    RVGP_CHECK_ARG(nb >= 0 && r >= 1, "rvgp_bsr_spmm: need nb >= 0 and r >= 1");
+    RVGP_CHECK_ARG(b >= 1 && b <= 3, "rvgp_bsr_spmm: block size must be 1, 2 or 3");
+    RVGP_CHECK_ARG(ldx >= r && ldy >= r, "rvgp_bsr_spmm: leading dimension < r");
+    RVGP_CHECK_ARG(Z == nullptr || ldz >= r, "rvgp_bsr_spmm: ldz < r");
+    RVGP_CHECK_ARG(acc == nullptr || ldacc >= r, "rvgp_bsr_spmm: ldacc < r");
+    RVGP_CHECK_ARG(X != Y, "rvgp_bsr_spmm: Y must not alias X");
+    if (nb == 0) return RVGP_OK;
+    hipStream_t s = as_stream(stream);
+    if (r <= 4) {
+        constexpr int LPR = 8;
+        dim3 g(cdiv((size_t)nb * LPR, 256));
+#define RVGP_NARROW(BB)                                                                         \
+    hipLaunchKernelGGL((spmm_narrow<T, BB, LPR, 4>), g, dim3(256), 0, s, nb, rowptr, cols, vals, \
+                       X, ldx, r, alpha, beta, Z, ldz, gamma, Y, ldy, acc, ldacc, delta)
+        if (b == 1) RVGP_NARROW(1);
+        else if (b == 2) RVGP_NARROW(2);
+        else RVGP_NARROW(3);
+#undef RVGP_NARROW
+    } else {
+        constexpr int CPL = sizeof(T) == 4 ? 4 : 2;
+        dim3 g(cdiv(nb, 4), cdiv(r, 64 * CPL));
+#define RVGP_WIDE(BB)                                                                          \
+    hipLaunchKernelGGL((spmm_wide<T, BB, CPL>), g, dim3(256), 0, s, nb, rowptr, cols, vals, X, \
+                       ldx, r, alpha, beta, Z, ldz, gamma, Y, ldy, acc, ldacc, delta)
+        if (b == 1) RVGP_WIDE(1);
+        else if (b == 2) RVGP_WIDE(2);
+        else RVGP_WIDE(3);
+#undef RVGP_WIDE
+    }
+    RVGP_LAUNCH_CHECK();
+    return RVGP_OK;
+}
+
+}  // namespace rvgp
+
+using namespace rvgp;
+
+extern "C" int rvgp_bsr_spmm_f64(int nb, int b, const int* rowptr, const int* cols,
+                                 const double* vals, const double* X, int ldx, int r,
+                                 double alpha, double beta, const double* Z, int ldz,
+                                 double gamma, double* Y, int ldy, double* acc, int ldacc,
+                                 double delta, void* stream) {
+    return spmm_dispatch<double>(nb, b, rowptr, cols, vals, X, ldx, r, alpha, beta, Z, ldz, gamma,
+                                 Y, ldy, acc, ldacc, delta, stream);
+}
+
+extern "C" int rvgp_bsr_spmm_f32(int nb, int b, const int* rowptr, const int* cols,
+                                 const float* vals, const float* X, int ldx, int r, float alpha,
+                                 float beta, const float* Z, int ldz, float gamma, float* Y,
+                                 int ldy, float* acc, int ldacc, float delta, void* stream) {
+    return spmm_dispatch<float>(nb, b, rowptr, cols, vals, X, ldx, r, alpha, beta, Z, ldz, gamma,
+                                Y, ldy, acc, ldacc, delta, stream);
+}

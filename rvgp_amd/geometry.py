@@ -1,0 +1,167 @@
+"""Drop-in for RVGP/geometry.py on MI355X.
+
+Every function keeps the reference's name, arguments, defaults and return
+types; the work runs in the HIP kernels of librvgp.so.
+"""
+import numpy as np
+import torch
+
+from . import _lib
+from .graph import SymGraph, as_device_graph
+
+
+def _dev_f64(x):
+    dev = _lib.device()
+    if isinstance(x, torch.Tensor):
+        return x.to(device=dev, dtype=torch.float64).contiguous()
+    return torch.as_tensor(np.ascontiguousarray(x, dtype=np.float64), device=dev)
+
+
+# ---------------------------------------------------------------------------
+# a1  geometry.py:126-162
+# ---------------------------------------------------------------------------
+def furthest_point_sampling_device(x, N=None, spacing=0.1, start_idx=0):
+    """Device tensors (perm int32, lambdas f64) of furthest_point_sampling."""
+    xd = _dev_f64(x)
+    n, D = xd.shape
+    n_iter = n if N is None else int(N)
+    if n_iter < 1:
+        raise IndexError("index 0 is out of bounds for axis 0 with size 0")
+    if not (0 <= int(start_idx) < n):
+        raise IndexError(f"index {start_idx} is out of bounds for axis 0 with size {n}")
+    dev = xd.device
+    s = _lib.stream()
+    diam2 = None
+    if N is None:
+        diam2 = torch.zeros(1, dtype=torch.float64, device=dev)
+        _lib.call("rvgp_fps_diam2", _lib.ptr(xd), n, D, _lib.ptr(diam2), s)
+    ds = torch.empty(n, dtype=torch.float64, device=dev)
+    perm = torch.empty(n_iter, dtype=torch.int32, device=dev)
+    lambdas = torch.empty(n_iter, dtype=torch.float64, device=dev)
+    nsel = torch.empty(1, dtype=torch.int32, device=dev)
+    _lib.call("rvgp_fps", _lib.ptr(xd), n, D, n_iter, int(start_idx), float(spacing),
+              1 if N is None else 0, _lib.ptr(diam2), _lib.ptr(ds), _lib.ptr(perm),
+              _lib.ptr(lambdas), _lib.ptr(nsel), s)
+    k = int(nsel.item())
+    return perm[:k], lambdas[:k]
+
+
+def furthest_point_sampling(x, N=None, spacing=0.1, start_idx=0):
+    """A greedy farthest-point order (geometry.py:126-162).
+
+    Returns (perm int32, lambdas f64) as numpy arrays, or
+    (arange(len(x)), None) when spacing == 0.0 (geometry.py:139-140).
+    """
+    if spacing == 0.0:
+        return np.arange(len(x)), None
+    perm, lambdas = furthest_point_sampling_device(x, N, spacing, start_idx)
+    return perm.cpu().numpy(), lambdas.cpu().numpy()
+
+
+# ---------------------------------------------------------------------------
+# a2  geometry.py:100-123
+# ---------------------------------------------------------------------------
+def knn_device(xd, k):
+    n, D = xd.shape
+    if not (0 < k < n):
+        raise ValueError(f"Expected n_neighbors < n_samples_fit, but n_neighbors = {k + 1}, "
+                         f"n_samples_fit = {n}, n_samples = {n}")
+    nbr = torch.empty((n, k), dtype=torch.int32, device=xd.device)
+    _lib.call("rvgp_knn", _lib.ptr(xd), n, D, int(k), _lib.ptr(nbr), _lib.stream())
+    return nbr
+
+
+def symmetrize_device(nbr):
+    n, k = nbr.shape
+    dev = nbr.device
+    ws_bytes = _lib.lib().rvgp_graph_symmetrize_ws_bytes(n, k)
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
+    rowptr = torch.empty(n + 1, dtype=torch.int32, device=dev)
+    cols = torch.empty(n * (2 * k + 1), dtype=torch.int32, device=dev)
+    _lib.call("rvgp_graph_symmetrize", _lib.ptr(nbr), n, k, _lib.ptr(rowptr), _lib.ptr(cols),
+              _lib.ptr(ws), ws_bytes, _lib.stream())
+    nnz = int(rowptr[n].item())
+    return rowptr, cols[:nnz].clone()
+
+
+def manifold_graph(X, typ='knn', n_neighbors=5):
+    """Fit graph over a pointset X (geometry.py:100-123).
+
+    Returns a :class:`SymGraph` -- the symmetrised CSR of kNN + I kept on the
+    device; ``.to_networkx()`` gives the networkx graph the reference returns.
+    """
+    if typ != 'knn':
+        raise NotImplementedError("typ='affinity' (geometry.py:114-118) is not on the GPU path")
+    xd = _dev_f64(X)
+    rowptr, cols = symmetrize_device(knn_device(xd, int(n_neighbors)))
+    pos = X if isinstance(X, np.ndarray) else None
+    return SymGraph(rowptr, cols, xd.shape[0], positions=pos)
+
+
+# ---------------------------------------------------------------------------
+# a6 / a7  geometry.py:14-63  (host scipy views of the device operators)
+# ---------------------------------------------------------------------------
+def compute_laplacian(G, normalization=False):
+    """L = D - A of the symmetrised graph, as scipy csr (geometry.py:55-63)."""
+    if normalization:
+        raise NotImplementedError("normalised Laplacian is not on the RVGP path")
+    import scipy.sparse as sp
+    g = as_device_graph(G)
+    A = g.to_scipy()
+    deg = np.asarray(A.sum(1)).ravel()
+    return sp.csr_matrix(sp.diags(deg) - A)
+
+
+def compute_connection_laplacian(G, R, normalization=None):
+    """kron(L, ones(d,d)) .multiply(R) as scipy csr (geometry.py:14-52)."""
+    import scipy.sparse as sp
+    n = len(G)
+    dim = R.shape[0] // n
+    L = compute_laplacian(G)
+    Lc = sp.csr_matrix(sp.kron(L, np.ones([dim, dim])).multiply(R))
+    if normalization == "rw":
+        deg = np.diff(as_device_graph(G).rowptr.cpu().numpy()) - 1.0
+        deg_inv = np.where(deg > 0, 1.0 / np.where(deg > 0, deg, 1.0), 0.0).repeat(dim)
+        Lc = sp.diags(deg_inv, 0, format='csr') @ Lc
+    return Lc
+
+
+# ---------------------------------------------------------------------------
+# a4  geometry.py:83-97
+# ---------------------------------------------------------------------------
+def manifold_dimension(Sigma, frac_explained=0.9):
+    """Estimate manifold dimension based on singular values (geometry.py:83-97).
+    Like the reference, squares and normalises ``Sigma`` in place."""
+    if isinstance(Sigma, torch.Tensor):
+        Sigma = Sigma.cpu().numpy()
+    if frac_explained == 1.0:
+        return Sigma.shape[1]
+    Sigma **= 2
+    Sigma /= Sigma.sum(1, keepdims=True)
+    Sigma = Sigma.cumsum(1)
+    var_exp = Sigma.mean(0) - Sigma.std(0)
+    dim_man = np.where(var_exp >= frac_explained)[0][0] + 1
+    print("Fraction of variance explained: ", var_exp)
+    return int(dim_man)
+
+
+# ---------------------------------------------------------------------------
+# a10 / a11 helpers  geometry.py:165-175 (host numpy, as in the reference)
+# ---------------------------------------------------------------------------
+def project_to_manifold(x, gauges):
+    """Project vectors to local coordinates over manifold (geometry.py:165-168)."""
+    coeffs = np.einsum("bij,bi->bj", gauges, x)
+    return np.einsum("bj,bij->bi", coeffs, gauges)
+
+
+def express_in_local_frame(x, gauges, reverse=False):
+    """Express vectors in local coordinates over manifold (geometry.py:171-175)."""
+    if reverse:
+        return np.einsum("bji,bi->bj", gauges, x)
+    return np.einsum("bij,bi->bj", gauges, x)
+
+
+def compute_spectrum(laplacian, n_eigenpairs=None, dtype=None):
+    """Smallest eigenpairs, eigenvectors scaled by sqrt(rows) (geometry.py:66-80)."""
+    from .eigsolve import spectrum_of_scipy
+    return spectrum_of_scipy(laplacian, n_eigenpairs)

@@ -1,0 +1,75 @@
+"""Block-CSR operators on the device and their fused SpMM.
+
+A :class:`BSR` holds the connection Laplacian (b = dim_man) or the scalar
+Laplacian (b = 1) in HBM: block-row pointer and block-column indices (int32,
+the symmetrised graph's CSR), and the b x b blocks row-major in f64 and f32.
+"""
+import numpy as np
+import torch
+
+from . import _lib
+
+
+class BSR:
+    def __init__(self, rowptr, cols, vals64, vals32, b):
+        self.rowptr, self.cols, self.b = rowptr, cols, int(b)
+        self.vals64, self.vals32 = vals64, vals32
+        self.nb = rowptr.numel() - 1
+        self.nnzb = cols.numel()
+        self.shape = (self.nb * self.b, self.nb * self.b)
+
+    def vals(self, dtype):
+        if dtype == torch.float64:
+            return self.vals64
+        if self.vals32 is None:
+            self.vals32 = self.vals64.to(torch.float32)
+        return self.vals32
+
+    def gershgorin_bound(self):
+        """max_i sum_j |A_ij| >= lambda_max (host scalar)."""
+        v = self.vals64.abs().reshape(self.nnzb, self.b, self.b).sum(2)  # (nnzb, b) row sums
+        rows = torch.repeat_interleave(torch.arange(self.nb, device=v.device),
+                                       torch.diff(self.rowptr.long()))
+        acc = torch.zeros(self.nb, self.b, dtype=torch.float64, device=v.device)
+        acc.index_add_(0, rows, v)
+        return float(acc.max().item())
+
+    def to_scipy(self):
+        import scipy.sparse as sp
+        rp, c = self.rowptr.cpu().numpy(), self.cols.cpu().numpy()
+        v = self.vals64.cpu().numpy().reshape(-1, self.b, self.b)
+        return sp.bsr_matrix((v, c, rp), shape=self.shape).tocsr()
+
+    @staticmethod
+    def from_scipy(A, b=1, device=None):
+        import scipy.sparse as sp
+        dev = device or _lib.device()
+        B = sp.bsr_matrix(A, blocksize=(b, b))
+        B.sort_indices()
+        rp = torch.as_tensor(B.indptr.astype(np.int32), device=dev)
+        c = torch.as_tensor(B.indices.astype(np.int32), device=dev)
+        v = torch.as_tensor(np.ascontiguousarray(B.data, dtype=np.float64), device=dev).reshape(-1)
+        return BSR(rp, c, v, None, b)
+
+
+def spmm(A, X, Y=None, alpha=1.0, beta=0.0, Z=None, gamma=0.0, acc=None, delta=0.0):
+    """Y = alpha A X + beta X + gamma Z ; acc += delta Y.  X is (nb*b, r) or
+    (nb*b,); dtype float64 or float32 selects the kernel and A's values."""
+    vec = X.dim() == 1
+    X2 = X.reshape(-1, 1) if vec else X
+    assert X2.shape[0] == A.shape[0], (X2.shape, A.shape)
+    r = X2.shape[1]
+    if X2.stride(1) != 1:
+        X2 = X2.contiguous()
+    if Y is None:
+        Y = torch.empty_like(X2)
+    Y2 = Y.reshape(-1, 1) if Y.dim() == 1 else Y
+    Z2 = None if Z is None else (Z.reshape(-1, 1) if Z.dim() == 1 else Z)
+    A2 = None if acc is None else (acc.reshape(-1, 1) if acc.dim() == 1 else acc)
+    name = "rvgp_bsr_spmm_f64" if X2.dtype == torch.float64 else "rvgp_bsr_spmm_f32"
+    vals = A.vals(X2.dtype)
+    _lib.call(name, A.nb, A.b, _lib.ptr(A.rowptr), _lib.ptr(A.cols), _lib.ptr(vals),
+              _lib.ptr(X2), X2.stride(0), r, float(alpha), float(beta), _lib.ptr(Z2),
+              Z2.stride(0) if Z2 is not None else r, float(gamma), _lib.ptr(Y2), Y2.stride(0),
+              _lib.ptr(A2), A2.stride(0) if A2 is not None else r, float(delta), _lib.stream())
+    return Y

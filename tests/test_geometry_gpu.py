@@ -1,0 +1,243 @@
+"""GPU parity of the geometry kernels (a1-a7) against the oracle.
+
+Bit-exact for indices (FPS order, kNN sets, CSR, heap neighbourhoods) and
+FPS lambdas; FP64 tolerances for SVD / polar outputs (gauge-invariant forms
+where singular-vector signs are free).
+"""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+from oracle import cpu
+from tests.util import sphere, torus, flat_torus3, golden_cases
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def geo():
+    from rvgp_amd import geometry
+    return geometry
+
+
+# ---------------------------------------------------------------- a1 FPS
+@pytest.mark.parametrize("n,D,spacing,N,start", [
+    (2000, 3, 0.1, None, 0),
+    (2000, 3, 0.05, None, 17),
+    (1500, 8, 0.2, None, 3),
+    (2000, 3, 0.1, 500, 0),
+    (300, 32, 0.1, 300, 5),
+])
+def test_fps_bit_exact(geo, n, D, spacing, N, start):
+    x = np.random.default_rng(n + D).normal(size=(n, D))
+    p_gpu, l_gpu = geo.furthest_point_sampling(x, N=N, spacing=spacing, start_idx=start)
+    p_ref, l_ref = cpu.fps(x, N=N, spacing=spacing, start_idx=start)
+    assert p_gpu.dtype == np.int32
+    np.testing.assert_array_equal(p_gpu, p_ref)
+    np.testing.assert_array_equal(l_gpu, l_ref)
+    # against the reference's own formula (sklearn pairwise_distances)
+    p_sk, l_sk = cpu.fps(x, N=N, spacing=spacing, start_idx=start, metric="sklearn")
+    np.testing.assert_array_equal(p_gpu, p_sk)
+    np.testing.assert_allclose(l_gpu, l_sk, rtol=1e-12, atol=1e-12)
+
+
+def test_fps_torus_c2_size(geo):
+    """Config-2 shape: 20,000 torus points, spacing 0.015."""
+    x = torus(20000, 0)
+    p_gpu, l_gpu = geo.furthest_point_sampling(x, spacing=0.015)
+    p_ref, l_ref = cpu.fps(x, spacing=0.015)
+    np.testing.assert_array_equal(p_gpu, p_ref)
+    np.testing.assert_array_equal(l_gpu, l_ref)
+    assert 3000 < len(p_gpu) < 5000
+    assert np.all(np.diff(l_gpu[1:]) <= 0)  # lambdas non-increasing
+    assert len(np.unique(p_gpu)) == len(p_gpu)
+
+
+def test_fps_edge_cases(geo):
+    x = np.random.default_rng(0).normal(size=(50, 3))
+    p, l = geo.furthest_point_sampling(x, spacing=0.0)
+    np.testing.assert_array_equal(p, np.arange(50))
+    assert l is None
+    # N > n: once every point is taken all distances are 0 and argmax -> 0
+    p, l = geo.furthest_point_sampling(x, N=60)
+    pr, lr = cpu.fps(x, N=60)
+    np.testing.assert_array_equal(p, pr)
+    np.testing.assert_array_equal(l, lr)
+    # exact ties: duplicated points, lowest index wins
+    xd = np.repeat(np.random.default_rng(1).normal(size=(20, 2)), 3, axis=0)
+    p, l = geo.furthest_point_sampling(xd, N=40)
+    pr, lr = cpu.fps(xd, N=40)
+    np.testing.assert_array_equal(p, pr)
+    # single point
+    p, l = geo.furthest_point_sampling(x[:1])
+    np.testing.assert_array_equal(p, [0])
+
+
+# ---------------------------------------------------------------- a2 kNN + CSR
+@pytest.mark.parametrize("n,D,k", [(2000, 3, 10), (3000, 32, 20), (1000, 8, 15), (500, 3, 40)])
+def test_knn_bit_exact(geo, n, D, k):
+    import torch
+    x = np.random.default_rng(7 * n + D).normal(size=(n, D))
+    xd = torch.as_tensor(x, device="cuda")
+    nbr = geo.knn_device(xd, k).cpu().numpy()
+    np.testing.assert_array_equal(nbr, cpu.knn(x, k))
+    rp, c = geo.symmetrize_device(geo.knn_device(xd, k))
+    rpo, co = cpu.sym_csr(cpu.knn(x, k))
+    np.testing.assert_array_equal(rp.cpu().numpy(), rpo)
+    np.testing.assert_array_equal(c.cpu().numpy(), co)
+
+
+def test_knn_ties_lattice(geo):
+    """Integer lattice: many exactly equal distances; lower index wins."""
+    import torch
+    g = np.stack(np.meshgrid(np.arange(12), np.arange(12), np.arange(3), indexing="ij"), -1)
+    x = g.reshape(-1, 3).astype(np.float64)
+    nbr = geo.knn_device(torch.as_tensor(x, device="cuda"), 6).cpu().numpy()
+    np.testing.assert_array_equal(nbr, cpu.knn(x, 6))
+
+
+def test_manifold_graph_matches_networkx(geo):
+    x = sphere(1200, 3)
+    G = geo.manifold_graph(x, n_neighbors=10)
+    rp, c, _ = cpu.graph_csr_networkx(x, 10)
+    np.testing.assert_array_equal(G.rowptr.cpu().numpy(), rp)
+    np.testing.assert_array_equal(G.cols.cpu().numpy(), c)
+    Gx = G.to_networkx()
+    assert Gx.number_of_nodes() == 1200 and all(Gx.has_edge(i, i) for i in range(1200))
+
+
+def test_knn_rejects_bad_k(geo):
+    with pytest.raises(ValueError):
+        geo.manifold_graph(np.zeros((5, 3)), n_neighbors=5)
+
+
+# ---------------------------------------------------------------- a3 heap nbh + gauges
+@pytest.mark.parametrize("case", golden_cases())
+def test_heap_neighbourhoods_and_gauges_vs_reference(case):
+    """Golden vectors written by the reference's own compiled ptu_dijkstra."""
+    import torch
+    from rvgp_amd import ptu_dijkstra
+    from rvgp_amd.graph import SymGraph
+    z = np.load(case)
+    X, k, K = z["X"], int(z["k"]), int(z["K"])
+    rp, c = cpu.sym_csr(cpu.knn(X, k))
+    g = SymGraph(torch.as_tensor(rp, device="cuda"), torch.as_tensor(c, device="cuda"), len(X))
+    nbh = ptu_dijkstra.geo_nbh_device(g, K).cpu().numpy()
+    nbh_ref, _ = cpu.geo_nbh(X, rp, c, K, centred=False)
+    np.testing.assert_array_equal(nbh, nbh_ref)
+    D = X.shape[1]
+    T, S = ptu_dijkstra.tangent_frames_device(torch.as_tensor(X, device="cuda"), g, K, D, D, D)
+    T, S = T.cpu().numpy(), S.cpu().numpy()
+    np.testing.assert_allclose(S, z["Sigma"], rtol=1e-9, atol=1e-12)
+    d = int(z["dim"])
+    P = np.einsum("nij,nkj->nik", T[:, :, :d], T[:, :, :d])
+    Pr = np.einsum("nij,nkj->nik", z["tangents"][:, :, :d], z["tangents"][:, :, :d])
+    assert np.abs(P - Pr).max() < 1e-9
+    # orthonormal frames
+    I = np.einsum("nji,njk->nik", T, T)
+    assert np.abs(I - np.eye(D)).max() < 1e-10
+
+
+def test_tangent_frames_dropin_api(golden_dir):
+    from rvgp_amd import ptu_dijkstra, geometry
+    z = np.load(os.path.join(golden_dir, "heap_sphere2000_k10.npz"))
+    X = z["X"]
+    G = geometry.manifold_graph(X, n_neighbors=10)
+    T, S = ptu_dijkstra.tangent_frames(X, G, X.shape[1], 10 * 1.5)
+    assert T.shape == (2000, 3, 3) and S.shape == (2000, 3)
+    np.testing.assert_allclose(S, z["Sigma"], rtol=1e-9, atol=1e-12)
+    with pytest.raises(ValueError):
+        ptu_dijkstra.tangent_frames(X, G, 3, 2000)
+    with pytest.raises(ValueError):
+        ptu_dijkstra.tangent_frames(X, G, 3, 2.0)
+    with pytest.raises(ValueError):
+        ptu_dijkstra.tangent_frames(X, G, 4, 15)
+
+
+def test_gauges_rank_deficient_raises():
+    """Planar points: the third singular value is 0 -> RuntimeError (pyx:426-428)."""
+    from rvgp_amd import ptu_dijkstra, geometry
+    rng = np.random.default_rng(0)
+    X = np.c_[rng.normal(size=(300, 2)), np.zeros(300)]
+    G = geometry.manifold_graph(X, n_neighbors=10)
+    with pytest.raises(RuntimeError):
+        ptu_dijkstra.tangent_frames(X, G, 3, 15)
+
+
+def test_gauges_wide_ambient_gram_path():
+    """D > K+1 (the 3-torus in R^32 shape): Gram-side eigensolve."""
+    import torch
+    from rvgp_amd import ptu_dijkstra
+    from rvgp_amd.graph import SymGraph
+    X = flat_torus3(600, 5, D=32)
+    k, K = 10, 15
+    rp, c = cpu.sym_csr(cpu.knn(X, k))
+    g = SymGraph(torch.as_tensor(rp, device="cuda"), torch.as_tensor(c, device="cuda"), len(X))
+    T, S = ptu_dijkstra.tangent_frames_device(torch.as_tensor(X, device="cuda"), g, K, K + 1, K + 1, K)
+    T, S = T.cpu().numpy(), S.cpu().numpy()
+    Tr, Sr = cpu.tangent_frames(X, rp, c, 32, K, allow_rank_deficient_k=True)
+    np.testing.assert_allclose(S[:, :K], Sr[:, :K], rtol=1e-8, atol=1e-10)
+    d = 3
+    P = np.einsum("nij,nkj->nik", T[:, :, :d], T[:, :, :d])
+    Pr = np.einsum("nij,nkj->nik", Tr[:, :, :d], Tr[:, :, :d])
+    assert np.abs(P - Pr).max() < 1e-8
+
+
+# ---------------------------------------------------------------- a5-a7 connections / Laplacians
+@pytest.mark.parametrize("case", golden_cases())
+def test_connections_vs_reference(case):
+    import torch
+    from rvgp_amd import ptu_dijkstra
+    from rvgp_amd.graph import SymGraph
+    z = np.load(case)
+    X, k, d = z["X"], int(z["k"]), int(z["dim"])
+    rp, c = cpu.sym_csr(cpu.knn(X, k))
+    g = SymGraph(torch.as_tensor(rp, device="cuda"), torch.as_tensor(c, device="cuda"), len(X))
+    # use the reference's own gauges so R is compared like for like
+    T = torch.as_tensor(np.ascontiguousarray(z["tangents"][:, :, :d]), device="cuda")
+    out = ptu_dijkstra.connections_device(T, g, d)
+    R = out["R"].cpu().numpy()
+    coo = ptu_dijkstra.blocks_to_coo(R, rp, c)
+    np.testing.assert_array_equal(coo.row, z["R_row"])
+    np.testing.assert_array_equal(coo.col, z["R_col"])
+    np.testing.assert_allclose(coo.data, z["R_data"], rtol=0, atol=1e-10)
+    # Laplacian blocks vs the restated scipy construction
+    Lc = cpu.connection_laplacian(rp, c, R)
+    from rvgp_amd.sparse import BSR
+    B = BSR(g.rowptr, g.cols, out["lc64"].reshape(-1), out["lc32"].reshape(-1), d)
+    assert abs(B.to_scipy() - Lc).max() < 1e-14
+    L = cpu.laplacian(rp, c)
+    assert np.abs(out["l64"].cpu().numpy() - L[np.repeat(np.arange(len(X)), np.diff(rp)), c].A1).max() == 0
+
+
+# ---------------------------------------------------------------- SpMM
+@pytest.mark.parametrize("b", [1, 2, 3])
+@pytest.mark.parametrize("r", [1, 3, 7, 130])
+def test_bsr_spmm(b, r):
+    import torch
+    from rvgp_amd.sparse import BSR, spmm
+    rng = np.random.default_rng(b * 100 + r)
+    x = sphere(700, b + r)
+    rp, c = cpu.sym_csr(cpu.knn(x, 8))
+    nnz = len(c)
+    vals = rng.normal(size=(nnz, b, b))
+    A = BSR(torch.as_tensor(rp, device="cuda"), torch.as_tensor(c, device="cuda"),
+            torch.as_tensor(vals.reshape(-1), device="cuda"), None, b)
+    As = A.to_scipy()
+    X = rng.normal(size=(700 * b, r))
+    Z = rng.normal(size=(700 * b, r))
+    ACC = rng.normal(size=(700 * b, r))
+    ref = 0.7 * (As @ X) - 0.3 * X + 1.5 * Z
+    for dt, tol in ((torch.float64, 1e-12), (torch.float32, 2e-5)):
+        Xd = torch.as_tensor(X, device="cuda", dtype=dt)
+        Zd = torch.as_tensor(Z, device="cuda", dtype=dt)
+        accd = torch.as_tensor(ACC, device="cuda", dtype=dt)
+        Y = spmm(A, Xd, alpha=0.7, beta=-0.3, Z=Zd, gamma=1.5, acc=accd, delta=0.25)
+        scale = np.abs(ref).max()
+        assert np.abs(Y.cpu().double().numpy() - ref).max() < tol * scale * 10
+        assert np.abs(accd.cpu().double().numpy() - (ACC + 0.25 * ref)).max() < tol * scale * 10
+        # plain product, in place over Z allowed
+        Y2 = spmm(A, Xd, Y=Zd, alpha=1.0, Z=Zd, gamma=0.0)
+        assert np.abs(Y2.cpu().double().numpy() - As @ X).max() < tol * np.abs(As @ X).max() * 10

@@ -147,6 +147,30 @@ int rvgp_bsr_spmm_f32(int nb, int b, const int* rowptr, const int* cols, const f
                       float delta, void* stream);
 
 /* ---------------------------------------------------------------------
+ * a10/a11  Vector-field frame algebra -- replaces geometry.project_to_manifold
+ *     and express_in_local_frame (RVGP/geometry.py:165-175, used at
+ *     dataclass.py:100, 110, 116) and the renormalisation of
+ *     smoothing.vector_diffusion (RVGP/smoothing.py:57-62).
+ * ------------------------------------------------------------------- */
+
+/* F fields over n nodes, frames T (n, D, ldt) f64 (first d columns used):
+ *   mode 0: out (F,n,d) = T^T in (F,n,D);  mode 1: out (F,n,D) = T in (F,n,d);
+ *   mode 2: out (F,n,D) = T T^T in (F,n,D)  (in == out allowed). */
+int rvgp_frame_apply(int mode, int F, int n, int D, int d, const double* T, int ldt,
+                     const double* in, double* out, void* stream);
+
+/* out (rows, d) with row stride ldo, in place:
+ *   out_i <- out_i * a_i / (b_i * |out_i|)   (a, b with strides lda, ldb). */
+int rvgp_diffusion_normalise(size_t rows, int d, double* out, int ldo, const double* a, int lda,
+                             const double* b, int ldb, void* stream);
+
+/* a9  Ambient eigenvector rows Phi = T_i U_i for the listed nodes
+ * (dataclass.py:57-59): out ((m*D), k) rows r*D+p, f32 if out_f32 else f64;
+ * U (n*d, k) f64 with row stride ldu. */
+int rvgp_phi_rows(int out_f32, int m, int D, int d, const double* T, int ldt, int k,
+                  const double* U, int ldu, const int* nodes, void* out, int ldo, void* stream);
+
+/* ---------------------------------------------------------------------
  * Dense linear algebra on the matrix cores.  Replaces the dense BLAS/LAPACK
  * inside ARPACK's Rayleigh-Ritz (geometry.py:73), the GPflow Gram
  * K = (X * S) X^T (kernels.py:55-61) and TensorFlow's Cholesky / triangular
@@ -179,6 +203,58 @@ int rvgp_trsm(int f64, int trans, int n, int r, const void* L, int lda, void* B,
 
 /* out[0] = sum_i log(L_ii) accumulated in f64 (half the log-determinant). */
 int rvgp_chol_logdet(int f64, int n, const void* L, int lda, double* out, void* stream);
+
+/* out[j] = sum_i Z[i][j]^2 (f64 accumulation, fixed order); Z (M, k) f64/f32. */
+size_t rvgp_colsumsq_ws_bytes(int k);
+int rvgp_colsumsq(int f64, int M, int k, const void* Z, int ldz, double* out, void* ws,
+                  size_t ws_bytes, void* stream);
+
+/* ---------------------------------------------------------------------
+ * a12/a14/a15  Spectral kernel, NLML and adjoint -- replaces
+ *     ManifoldKernel.eval_S / K / K_diag (RVGP/kernels.py:41-67) and GPflow's
+ *     GPR.training_loss with TF autodiff under the scipy optimiser
+ *     (RVGP/main.py:87-95).  Parameters theta = (nu, kappa, sigma_f, s2),
+ *     theta = softplus(u) + lower.  All scalars live on the device so an
+ *     optimiser step can be captured in a HIP graph.
+ * ------------------------------------------------------------------- */
+
+/* S (k) and dS (3, k) = dS/d(nu, kappa, sigma_f) of the spectral filter,
+ * typ 0 = matern, 1 = se; theta (3,) device; n_rows = rows of evecs_Lc. */
+int rvgp_spectral_filter(int typ, int k, const double* evals, const double* theta, double n_rows,
+                         double* S, double* dS, void* stream);
+
+/* A = G + diag(s2 / S)  (weight-space system matrix), s2 a device scalar. */
+int rvgp_ws_form(int k, const double* G, int ldg, const double* S, const double* s2, double* A,
+                 int lda, void* stream);
+
+/* Weight-space NLML and gradients.  out (9,) = [nlml, d/d(nu,kappa,sigma_f,s2),
+ * d/du(...)]; gS: scratch (k,).  M = number of observations. */
+int rvgp_ws_finish(int k, double M, const double* L, int ldl, const double* ainv_diag,
+                   const double* b, const double* w, const double* Gw, const double* yy,
+                   const double* S, const double* dS, const double* s2, const double* u, double* gS,
+                   double* out, void* stream);
+
+/* Function-space NLML and gradients from q_j = |L^-1 phi_j|^2, p_j = phi_j^T a,
+ * ya = y^T a, aa = a^T a and logdet_half = sum log L_ii (all device). */
+int rvgp_fs_finish(int k, double M, const double* q, const double* p, const double* ya,
+                   const double* aa, const double* logdet_half, const double* S, const double* dS,
+                   const double* s2, const double* u, double* gS, double* out, void* stream);
+
+/* Determinant-lemma terms for the mixed-precision function-space form, from
+ * the FP64 factor LA of A = s2 S^-1 + Phi^T Phi and diag(A^-1):
+ * q_j = phi_j^T K_y^-1 phi_j and logdet_half = log det(K_y) / 2. */
+int rvgp_lemma_terms(int k, double M, const double* S, const double* s2, const double* ainv_diag,
+                     const double* LA, int lda, double* q, double* logdet_half, void* stream);
+
+/* theta = softplus(u) + lower: kernel_theta (3,) and s2 (1,). */
+int rvgp_gp_params(const double* u, const double* lower, double* kernel_theta, double* s2,
+                   void* stream);
+
+/* One Adam step on the 4 unconstrained parameters; state m, v (4,), t (1,);
+ * trainable (4,) int flags. */
+int rvgp_adam_step(double* u, const double* g, double* m, double* v, double* t,
+                   const int* trainable, double lr, double b1, double b2, double eps,
+                   void* stream);
 
 #ifdef __cplusplus
 }

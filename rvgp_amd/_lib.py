@@ -36,6 +36,22 @@ SIGNATURES = {
     "rvgp_connections": (I, [P, I, I, I, I, P, P, I, P, P, P, P, P, P]),
     "rvgp_bsr_spmm_f64": (I, [I, I, P, P, P, P, I, I, D, D, P, I, D, P, I, P, I, D, P]),
     "rvgp_bsr_spmm_f32": (I, [I, I, P, P, P, P, I, I, F, F, P, I, F, P, I, P, I, F, P]),
+    "rvgp_gemm": (I, [I, I, I, I, I, I, D, P, I, P, I, P, D, P, I, I, P, S, P]),
+    "rvgp_potrf": (I, [I, I, P, I, P, P]),
+    "rvgp_trsm": (I, [I, I, I, I, P, I, P, I, P]),
+    "rvgp_chol_logdet": (I, [I, I, P, I, P, P]),
+    "rvgp_frame_apply": (I, [I, I, I, I, I, P, I, P, P, P]),
+    "rvgp_diffusion_normalise": (I, [S, I, P, I, P, I, P, I, P]),
+    "rvgp_phi_rows": (I, [I, I, I, I, P, I, I, P, I, P, P, I, P]),
+    "rvgp_colsumsq_ws_bytes": (S, [I]),
+    "rvgp_colsumsq": (I, [I, I, I, P, I, P, P, S, P]),
+    "rvgp_spectral_filter": (I, [I, I, P, P, D, P, P, P]),
+    "rvgp_ws_form": (I, [I, P, I, P, P, P, I, P]),
+    "rvgp_ws_finish": (I, [I, D, P, I, P, P, P, P, P, P, P, P, P, P, P, P]),
+    "rvgp_fs_finish": (I, [I, D, P, P, P, P, P, P, P, P, P, P, P, P]),
+    "rvgp_lemma_terms": (I, [I, D, P, P, P, P, I, P, P, P]),
+    "rvgp_gp_params": (I, [P, P, P, P, P]),
+    "rvgp_adam_step": (I, [P, P, P, P, P, P, D, D, D, D, P]),
 }
 
 RVGP_EINVAL, RVGP_ERUNTIME, RVGP_ENOMEM, RVGP_EHIP = -1, -2, -3, -4

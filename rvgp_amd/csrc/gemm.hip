@@ -164,14 +164,17 @@ __global__ __launch_bounds__(GT) void gemm_kernel(int transA, int transB, int M,
         }
         __syncthreads();
     }
-    // epilogue: lane holds rows 4*(lane/16)+r, column lane%16 of each 16x16 tile
+    // epilogue: lane holds column lane%16 of each 16x16 tile, in rows
+    //   f32 / bf16 MFMA: 4*(lane/16) + r       (four consecutive rows)
+    //   f64 MFMA:        lane/16 + 4*r         (rows interleaved by 4)
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int gm = m0 + wm + i * 16 + 4 * (lane >> 4) + r;
+                const int rr = (sizeof(Lds) == 8) ? (lane >> 4) + 4 * r : 4 * (lane >> 4) + r;
+                const int gm = m0 + wm + i * 16 + rr;
                 const int gn = n0 + wn + j * 16 + (lane & 15);
                 if (gm < M && gn < N) {
                     const Acc v = acc[i][j][r];

@@ -1,0 +1,258 @@
+"""Drop-in for RVGP/dataclass.py (``RVGP.create_data_object``).
+
+The constructor runs the whole geometry path on the GPU -- kNN graph,
+heap-exact neighbourhoods and gauges, manifold dimension, connections,
+block-CSR Laplacians and both eigensolves (dataclass.py:22-89) -- and keeps
+every array in HBM.  The reference's attributes (vertices, n, dim_man, G,
+gauges, R, L, Lc, evals_L, evecs_L, evals_Lc, evecs_Lc, vectors) are
+available with the reference's types (numpy / scipy / networkx); they are
+materialised on the host only when read, and assigning to them (as the
+reference's examples do, e.g. truncating evecs_Lc) is honoured by fit().
+"""
+import math
+
+import numpy as np
+import torch
+
+from . import _lib, geometry
+from .eigsolve import chfsi
+from .ptu_dijkstra import connections_device, tangent_frames_device
+from .smoothing import vector_diffusion_device
+from .sparse import BSR
+
+
+class data:
+    def __init__(self, vertices, vectors=None, dim_man=2, n_neighbors=10,
+                 frac_geodesic_neighbours=1.5, explained_variance=0.8, n_eigenpairs=None, *,
+                 precision="fp64", reference_semantics=True, eig_tol=1e-9, verbose=True,
+                 timings=None):
+        """precision: "fp64" (all FP64) or "mixed" (FP32 eigensolver body with
+        FP64 Rayleigh-Ritz and polish).  reference_semantics=False enables
+        the documented extension for D > K (min(D, K+1) singular values)."""
+        t = timings if timings is not None else {}
+        sync = torch.cuda.synchronize
+        import time
+        t0 = time.perf_counter()
+        if verbose:
+            print('Fit graph')
+        Xd = geometry._dev_f64(vertices)
+        N, D = Xd.shape
+        G = geometry.manifold_graph(Xd, n_neighbors=n_neighbors)
+        if isinstance(vertices, np.ndarray):
+            G._positions = vertices
+        sync()
+        t["graph"] = time.perf_counter() - t0
+
+        if verbose:
+            print('Fit tangent spaces')
+        t0 = time.perf_counter()
+        K = n_neighbors * frac_geodesic_neighbours
+        if K >= N:
+            raise ValueError("Geodesic neighborhood size must be less than the total number of samples")
+        if K < D and reference_semantics:
+            raise ValueError("Geodesic neighborhood size must be larger or equal to the embedding dimension")
+        K_t = int(K)
+        mn = min(D, K_t + 1)
+        n_check = mn if reference_semantics else min(D, K_t)
+        tangents, Sigma = tangent_frames_device(Xd, G, K_t, mn, mn, n_check)
+        sync()
+        t["gauges"] = time.perf_counter() - t0
+        dm = geometry.manifold_dimension(Sigma.cpu().numpy(), frac_explained=explained_variance) \
+            if verbose else _quiet_dimension(Sigma, explained_variance)
+        gauges = tangents[:, :, :dm].contiguous()
+        if verbose:
+            print('Predicted manifold dimension is {}'.format(dm))
+
+        t0 = time.perf_counter()
+        conn = connections_device(gauges, G, dm)
+        sync()
+        t["connections"] = time.perf_counter() - t0
+        if verbose:
+            print('Fit connections')
+            print('Compute Laplacians')
+        Lb = BSR(G.rowptr, G.cols, conn["l64"], conn["l32"], 1)
+        Lcb = BSR(G.rowptr, G.cols, conn["lc64"].reshape(-1), conn["lc32"].reshape(-1), dm)
+
+        if verbose:
+            print('Compute eigendecompositions')
+        t0 = time.perf_counter()
+        kL = N if n_eigenpairs is None else min(int(n_eigenpairs), N)
+        kLc = N * dm if n_eigenpairs is None else min(int(n_eigenpairs), N * dm)
+        wL, VL, stL = chfsi(Lb, kL, precision=precision, tol=eig_tol)
+        sync()
+        t["eig_L"] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        wLc, VLc, stLc = chfsi(Lcb, kLc, precision=precision, tol=eig_tol)
+        sync()
+        t["eig_Lc"] = time.perf_counter() - t0
+        self.eig_stats = {"L": stL, "Lc": stLc}
+
+        self._X = Xd
+        self._vertices_in = vertices
+        self.n = N
+        self.dim_man = dm
+        self._graph = G
+        self._gauges = gauges
+        self._tangents_full = tangents
+        self._R = conn["R"]
+        self._Lb, self._Lcb = Lb, Lcb
+        self._evals_L = wL
+        self._evecs_L = VL * math.sqrt(N)          # geometry.py:75
+        self._evals_Lc = wLc
+        self._U = VLc * math.sqrt(N * dm)          # intrinsic (n*d, k)
+        self.n_eigenpairs = kLc
+        self._host = {}
+        self._vectors = None
+        self.vectors = vectors
+
+    # ------------------------------------------------------------------ attributes
+    @property
+    def vertices(self):
+        return self._vertices_in
+
+    @property
+    def G(self):
+        return self._graph.to_networkx()
+
+    @property
+    def gauges(self):
+        if "gauges" not in self._host:
+            self._host["gauges"] = self._gauges.cpu().numpy()
+        return self._host["gauges"]
+
+    @property
+    def R(self):
+        if "R" not in self._host:
+            from .ptu_dijkstra import blocks_to_coo
+            rp, c = self._graph.csr_numpy()
+            self._host["R"] = blocks_to_coo(self._R.cpu().numpy(), rp, c)
+        return self._host["R"]
+
+    @property
+    def L(self):
+        if "L" not in self._host:
+            self._host["L"] = self._Lb.to_scipy()
+        return self._host["L"]
+
+    @property
+    def Lc(self):
+        if "Lc" not in self._host:
+            self._host["Lc"] = self._Lcb.to_scipy()
+        return self._host["Lc"]
+
+    def _getset(name, dev_attr):
+        def get(self):
+            if name not in self._host:
+                self._host[name] = getattr(self, dev_attr).cpu().numpy()
+            return self._host[name]
+
+        def set_(self, v):
+            self._host[name] = np.asarray(v)
+            self._overrides = getattr(self, "_overrides", set()) | {name}
+        return property(get, set_)
+
+    evals_L = _getset("evals_L", "_evals_L")
+    evecs_L = _getset("evecs_L", "_evecs_L")
+    evals_Lc = _getset("evals_Lc", "_evals_Lc")
+
+    @property
+    def evecs_Lc(self):
+        """Ambient eigenvectors Phi = gauges U, shape (n*D, k) (dataclass.py:57-59)."""
+        if "evecs_Lc" not in self._host:
+            nodes = torch.arange(self.n, dtype=torch.int32, device=self._X.device)
+            self._host["evecs_Lc"] = self.phi_rows(nodes).cpu().numpy()
+        return self._host["evecs_Lc"]
+
+    @evecs_Lc.setter
+    def evecs_Lc(self, v):
+        self._host["evecs_Lc"] = np.asarray(v)
+        self._overrides = getattr(self, "_overrides", set()) | {"evecs_Lc"}
+
+    del _getset
+
+    def _overridden(self, name):
+        return name in getattr(self, "_overrides", set())
+
+    def spectrum_device(self):
+        """(evals (k,), device) honouring assignments to evals_Lc."""
+        if self._overridden("evals_Lc"):
+            return torch.as_tensor(np.asarray(self._host["evals_Lc"], np.float64), device=self._X.device)
+        return self._evals_Lc
+
+    def phi_rows(self, nodes, dtype=torch.float64):
+        """Rows of evecs_Lc for the given nodes, (len(nodes)*D, k) on device."""
+        D = self._X.shape[1]
+        if self._overridden("evecs_Lc"):
+            E = np.asarray(self._host["evecs_Lc"]).reshape(self.n, -1)
+            idx = nodes.cpu().numpy() if isinstance(nodes, torch.Tensor) else np.asarray(nodes)
+            rows = E[idx].reshape(-1, E.shape[1] // D)
+            return torch.as_tensor(np.ascontiguousarray(rows), device=self._X.device).to(dtype)
+        nodes = torch.as_tensor(nodes, dtype=torch.int32, device=self._X.device).contiguous()
+        m = nodes.numel()
+        k = self._U.shape[1]
+        out = torch.empty((m * D, k), dtype=dtype, device=self._X.device)
+        T = self._gauges
+        _lib.call("rvgp_phi_rows", int(dtype == torch.float32), m, D, self.dim_man, _lib.ptr(T),
+                  T.shape[2], k, _lib.ptr(self._U), self._U.stride(0), _lib.ptr(nodes),
+                  _lib.ptr(out), k, _lib.stream())
+        return out
+
+    @property
+    def vectors(self):
+        if self._vectors is None:
+            return None
+        if "vectors" not in self._host:
+            self._host["vectors"] = self._vectors.cpu().numpy()
+        return self._host["vectors"]
+
+    @vectors.setter
+    def vectors(self, v):
+        self._host.pop("vectors", None)
+        if v is None:
+            self._vectors = None
+        elif isinstance(v, torch.Tensor):
+            self._vectors = v.to(self._X.device, torch.float64).contiguous()
+        else:
+            self._vectors = torch.as_tensor(np.ascontiguousarray(v, dtype=np.float64), device=self._X.device)
+
+    def vectors_device(self):
+        return self._vectors
+
+    # ------------------------------------------------------------------ fields
+    def _frame(self, mode, v, out_cols):
+        F = v.shape[0] if v.dim() == 3 else 1
+        out = torch.empty((F, self.n, out_cols), dtype=torch.float64, device=v.device)
+        T = self._gauges
+        _lib.call("rvgp_frame_apply", mode, F, self.n, T.shape[1], self.dim_man, _lib.ptr(T),
+                  T.shape[2], _lib.ptr(v.contiguous()), _lib.ptr(out), _lib.stream())
+        return out if v.dim() == 3 else out[0]
+
+    def random_vector_field(self, seed=0):
+        """Generate random vector field over manifold (dataclass.py:91-103).
+        Uses numpy's legacy RandomState stream exactly as the reference."""
+        np.random.seed(seed)
+        v = np.random.uniform(low=-0.5, high=0.5, size=(self.n, self._X.shape[1]))
+        vd = torch.as_tensor(v, device=self._X.device)
+        vd = self._frame(2, vd, self._X.shape[1])
+        vd = vd / torch.linalg.vector_norm(vd, dim=1, keepdim=True)
+        self.vectors = vd
+
+    def smooth_vector_field(self, t=100):
+        """Vector diffusion smoothing (dataclass.py:105-120).  vectors may be
+        (n, D) or a batch (F, n, D) of fields smoothed in one pass."""
+        if self._vectors is None:
+            raise TypeError("No vectors to smooth: data.vectors is None")
+        c = self._frame(0, self._vectors, self.dim_man)
+        c = vector_diffusion_device(c, t, self._Lcb, self._Lb)
+        self.vectors = self._frame(1, c, self._X.shape[1])
+
+
+def _quiet_dimension(Sigma, frac):
+    S = Sigma.cpu().numpy().copy()
+    if frac == 1.0:
+        return S.shape[1]
+    S **= 2
+    S /= S.sum(1, keepdims=True)
+    S = S.cumsum(1)
+    var_exp = S.mean(0) - S.std(0)
+    return int(np.where(var_exp >= frac)[0][0] + 1)

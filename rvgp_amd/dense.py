@@ -1,0 +1,85 @@
+"""Host wrappers of the matrix-core GEMM, Cholesky and triangular solves."""
+import torch
+
+from . import _lib
+
+GEMM_F64, GEMM_F32, GEMM_F32_ACC64, GEMM_BF16 = 0, 1, 2, 3
+
+
+class CholeskyError(RuntimeError):
+    """Raised when a matrix handed to the Cholesky factorisation is not
+    positive definite (tf.linalg.cholesky's failure inside GPflow)."""
+
+
+def _ld(t):
+    assert t.dim() == 2 and t.stride(1) == 1, "row-major 2-D tensor with unit column stride"
+    return t.stride(0)
+
+
+def gemm(A, B, C=None, transA=False, transB=False, alpha=1.0, beta=0.0, kscale=None,
+         mode=None, split_k=None, out_dtype=None):
+    """C = alpha op(A) diag(kscale) op(B) + beta C on the matrix cores."""
+    M = A.shape[1] if transA else A.shape[0]
+    K = A.shape[0] if transA else A.shape[1]
+    Kb = B.shape[1] if transB else B.shape[0]
+    N = B.shape[0] if transB else B.shape[1]
+    assert K == Kb, (A.shape, B.shape, transA, transB)
+    if mode is None:
+        if A.dtype == torch.float64:
+            mode = GEMM_F64
+        elif out_dtype == torch.float64 or (C is not None and C.dtype == torch.float64):
+            mode = GEMM_F32_ACC64
+        else:
+            mode = GEMM_F32
+    want = {GEMM_F64: (torch.float64, torch.float64), GEMM_F32: (torch.float32, torch.float32),
+            GEMM_F32_ACC64: (torch.float32, torch.float64),
+            GEMM_BF16: (torch.float32, torch.float32)}[mode]
+    assert A.dtype == want[0] and B.dtype == want[0], (A.dtype, B.dtype, mode)
+    if C is None:
+        C = torch.empty((M, N), dtype=want[1], device=A.device)
+        beta = 0.0
+    assert C.dtype == want[1] and C.shape == (M, N)
+    if kscale is not None:
+        kscale = kscale.to(torch.float64).contiguous()
+    tiles = -(-M // 64) * -(-N // 64)
+    if split_k is None:
+        split_k = 1
+        if tiles < 1024 and K >= 2048:
+            split_k = int(min(max(1, 2048 // tiles), K // 512, 256))
+    ws = None
+    ws_bytes = 0
+    if split_k > 1:
+        ws_bytes = split_k * M * N * C.element_size()
+        ws = torch.empty(ws_bytes, dtype=torch.uint8, device=A.device)
+    _lib.call("rvgp_gemm", mode, int(transA), int(transB), M, N, K, float(alpha), _lib.ptr(A),
+              _ld(A), _lib.ptr(B), _ld(B), _lib.ptr(kscale), float(beta), _lib.ptr(C), _ld(C),
+              int(split_k), _lib.ptr(ws), ws_bytes, _lib.stream())
+    return C
+
+
+def potrf(A, check=True):
+    """In-place lower Cholesky of a square row-major tensor (f64 or f32)."""
+    n = A.shape[0]
+    st = torch.zeros(1, dtype=torch.int32, device=A.device)
+    _lib.call("rvgp_potrf", int(A.dtype == torch.float64), n, _lib.ptr(A), _ld(A), _lib.ptr(st),
+              _lib.stream())
+    if check and int(st.item()) != 0:
+        raise CholeskyError("Cholesky decomposition was not successful. The input might not be "
+                            "valid (matrix not positive definite).")
+    return A, st
+
+
+def trsm(L, B, trans=False):
+    """In place: B <- L^{-1} B (trans=False) or L^{-T} B (trans=True)."""
+    B2 = B.reshape(-1, 1) if B.dim() == 1 else B
+    _lib.call("rvgp_trsm", int(L.dtype == torch.float64), int(trans), L.shape[0], B2.shape[1],
+              _lib.ptr(L), _ld(L), _lib.ptr(B2), _ld(B2), _lib.stream())
+    return B
+
+
+def chol_logdet_half(L):
+    """sum_i log L_ii (f64 device scalar)."""
+    out = torch.empty(1, dtype=torch.float64, device=L.device)
+    _lib.call("rvgp_chol_logdet", int(L.dtype == torch.float64), L.shape[0], _lib.ptr(L), _ld(L),
+              _lib.ptr(out), _lib.stream())
+    return out

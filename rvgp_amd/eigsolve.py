@@ -1,0 +1,228 @@
+"""Smallest eigenpairs of the (connection) Laplacian on the GPU.
+
+Replaces ``scipy.sparse.linalg.eigsh(L, k, which="SM")`` of
+RVGP/geometry.py:66-80 (ARPACK Lanczos without shift-invert).
+
+Method: a block Chebyshev-filtered subspace iteration (Zhou & Saad 2007,
+"scaled" filter) with a Rayleigh-Ritz step.  Each iteration
+
+  1. filters the current block X through a degree-m Chebyshev polynomial of
+     A that is ~1 on [0, cut] and damped on [cut, ub] -- m fused BSR SpMMs
+     Y <- (2s/e)(A - c) Y - s s' Y_prev (rvgp_bsr_spmm), the only work that
+     touches the n*d-sized operator;
+  2. Rayleigh-Ritz in FP64: G = Y^T Y and H = Y^T (A Y) on the matrix cores
+     (rvgp_gemm, split-K), G = V diag(g) V^T, B = V g^{-1/2} (drops
+     numerically dependent directions instead of failing like CholQR would),
+     eig(B^T H B) = Q diag(theta) Q^T, X = Y B Q and AX = (A Y) B Q;
+  3. residuals |AX - X theta| per column decide convergence.
+
+precision="fp64": everything in FP64 (the plain FP64 GPU form).
+precision="mixed": the filter and the tall products run on FP32 blocks with
+the Grams accumulated in FP64 (f32 -> f64 MFMA); once the FP32 residuals
+stall at FP32 level, FP64 polish iterations (FP64 SpMM + FP64 RR) bring the
+wanted pairs to the FP64 tolerance.  The small projected problems (p x p,
+p = k + guard) are solved on the GPU in FP64 (torch.linalg.eigh).
+
+Returns eigenvalues ascending and eigenvectors with unit 2-norm; the callers
+scale by sqrt(rows) as geometry.py:75 does.
+"""
+import math
+import time
+
+import numpy as np
+import torch
+
+from . import dense
+from .sparse import BSR, spmm
+
+
+class EigStats(dict):
+    pass
+
+
+def _rayleigh_ritz(A, Y, work_dtype):
+    """Returns X (N x p'), theta (p',) f64, AX (N x p') in work dtype."""
+    W = spmm(A, Y)
+    G = dense.gemm(Y, Y, transA=True, out_dtype=torch.float64)
+    H = dense.gemm(Y, W, transA=True, out_dtype=torch.float64)
+    G = 0.5 * (G + G.T)
+    H = 0.5 * (H + H.T)
+    g, V = torch.linalg.eigh(G)
+    gmax = float(g[-1].item())
+    drop = 1e-13 if work_dtype == torch.float64 else 1e-10
+    keep = g > gmax * drop
+    V = V[:, keep]
+    Bm = V / torch.sqrt(g[keep])
+    Hb = Bm.T @ H @ Bm
+    Hb = 0.5 * (Hb + Hb.T)
+    theta, Q = torch.linalg.eigh(Hb)
+    Cm = (Bm @ Q).to(work_dtype).contiguous()
+    X = dense.gemm(Y, Cm)
+    AX = dense.gemm(W, Cm)
+    return X, theta, AX
+
+
+def _residuals(X, theta, AX):
+    R = AX - X * theta.to(X.dtype)
+    rn = torch.linalg.vector_norm(R.double(), dim=0)
+    xn = torch.linalg.vector_norm(X.double(), dim=0)
+    return rn / xn
+
+
+def _cheb_filter(A, X, m, a, b, a0):
+    """Scaled Chebyshev filter of degree m (X is overwritten)."""
+    e = (b - a) / 2.0
+    c = (b + a) / 2.0
+    sigma = e / (a0 - c)
+    tau = 2.0 / sigma
+    Y = spmm(A, X, alpha=sigma / e, beta=-sigma * c / e)
+    prev, cur = X, Y
+    s1 = sigma
+    for _ in range(2, m + 1):
+        s2 = 1.0 / (tau - s1)
+        spmm(A, cur, Y=prev, alpha=2.0 * s2 / e, beta=-2.0 * s2 * c / e, Z=prev, gamma=-s1 * s2)
+        prev, cur = cur, prev
+        s1 = s2
+    return cur
+
+
+def _degree(theta, k, cut, ub, max_ratio, m_min=4, m_max=60):
+    """Largest degree whose amplification spread over the block stays below
+    max_ratio (keeps the filtered block well enough conditioned)."""
+    e = (ub - cut) / 2.0
+    c = (ub + cut) / 2.0
+    x0 = (c - float(theta[0])) / e
+    xk = (c - float(theta[min(k, len(theta)) - 1])) / e
+    a0 = math.acosh(max(x0, 1.0 + 1e-15))
+    ak = math.acosh(max(xk, 1.0 + 1e-15))
+    if a0 - ak <= 1e-12:
+        return m_max
+    m = int(math.log(max_ratio) / (a0 - ak))
+    return max(m_min, min(m_max, m))
+
+
+def _deflate(Y, Xl):
+    """Y <- Y - Xl (Xl^T Y), twice (classical Gram-Schmidt with reorthogonalisation)."""
+    if Xl is None or Xl.shape[1] == 0:
+        return Y
+    for _ in range(2):
+        C = dense.gemm(Xl, Y, transA=True, out_dtype=torch.float64).to(Y.dtype).contiguous()
+        dense.gemm(Xl, C, Y, alpha=-1.0, beta=1.0)
+    return Y
+
+
+def _run_phase(A, X, k, ub, wdt, ptol, max_iter, max_ratio, lock, fresh, stats, verbose, tag):
+    """Filter + Rayleigh-Ritz iterations on block X (work dtype wdt) until the
+    k wanted pairs have residual/ub < ptol.  With ``lock``, converged leading
+    pairs are frozen and deflated out of the filtered block."""
+    p = X.shape[1]
+    Xl = X[:, :0]
+    thl = torch.zeros(0, dtype=torch.float64, device=X.device)
+    Y = X
+    it = 0
+    while True:
+        tr = time.perf_counter()
+        Y = _deflate(Y, Xl)
+        Xa, tha, AXa = _rayleigh_ritz(A, Y, wdt)
+        stats["rr"] += 1
+        stats["spmm"] += 1
+        stats["t_rr"] += time.perf_counter() - tr
+        res = _residuals(Xa, tha, AXa) / ub
+        need = k - Xl.shape[1]
+        kk = min(need, res.numel())
+        worst = float(res[:kk].max().item()) if kk > 0 else 0.0
+        if verbose:
+            print(f"[chfsi {tag}] it={it} locked={Xl.shape[1]} active={Xa.shape[1]} "
+                  f"worst_res={worst:.3e}")
+        if kk >= need and worst < ptol:
+            X = torch.cat([Xl, Xa], 1)
+            return X, torch.cat([thl, tha]), it
+        if it >= max_iter:
+            X = torch.cat([Xl, Xa], 1)
+            stats["unconverged"] = worst
+            return X, torch.cat([thl, tha]), it
+        if lock:
+            conv = (res < ptol).cpu().numpy()
+            c = 0
+            while c < kk and conv[c]:
+                c += 1
+            c = min(c, max(0, Xa.shape[1] - 8))  # keep a working block
+            if c > 0:
+                Xl = torch.cat([Xl, Xa[:, :c]], 1).contiguous()
+                thl = torch.cat([thl, tha[:c]])
+                Xa, tha = Xa[:, c:].contiguous(), tha[c:]
+        # keep the total width p: top the active block up with random columns
+        if Xl.shape[1] + Xa.shape[1] < p:
+            Xa = torch.cat([Xa, fresh(p - Xl.shape[1] - Xa.shape[1], wdt)], 1).contiguous()
+        cut = float(tha[min(tha.numel(), Xa.shape[1]) - 1].item())
+        low = float(tha[0].item())
+        kk = min(k - Xl.shape[1], tha.numel())
+        m = _degree(tha, max(kk, 1), cut, ub, max_ratio)
+        tf = time.perf_counter()
+        Y = _cheb_filter(A, Xa, m, cut, ub, low)
+        torch.cuda.synchronize()
+        stats["t_filter"] += time.perf_counter() - tf
+        stats["spmm"] += m
+        stats["iters"] += 1
+        it += 1
+
+
+def chfsi(A: BSR, k, precision="fp64", tol=1e-9, max_iter=300, guard=None, seed=0,
+          verbose=False, X0=None):
+    """k smallest eigenpairs of the symmetric PSD operator A."""
+    N = A.shape[0]
+    if k >= N - 1:
+        raise TypeError("Cannot use scipy.linalg.eigh for sparse A with k >= N - 1. "
+                        "Use scipy.linalg.eigh(A.toarray()) or reduce k.")
+    if k < 1:
+        raise ValueError("k must be positive")
+    dev = A.rowptr.device
+    if guard is None:
+        guard = max(8, int(math.ceil(0.15 * k)))
+    p = min(N, k + guard)
+    ub = A.gershgorin_bound() * 1.0001 + 1e-12
+    stats = EigStats(spmm=0, rr=0, iters=0, t_filter=0.0, t_rr=0.0)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(int(seed))
+
+    def fresh(cols, dtype):
+        return torch.randn((N, cols), generator=gen, device=dev, dtype=torch.float64).to(dtype)
+
+    t0 = time.perf_counter()
+    X = X0.contiguous() if X0 is not None else fresh(p, torch.float64)
+    if precision == "mixed":
+        # FP32 filter body with locking, to FP32-level residuals
+        X, theta, it = _run_phase(A, X.to(torch.float32).contiguous(), k, ub, torch.float32, 2e-6,
+                                  max_iter, 300.0, True, fresh, stats, verbose, "fp32")
+        stats["iters_fp32"] = it
+        stats["t_fp32"] = time.perf_counter() - t0
+        # FP64 polish of the whole block (no locking): a few filter + RR passes
+        X, theta, it = _run_phase(A, X.double().contiguous(), k, ub, torch.float64, tol, 50, 1e4,
+                                  False, fresh, stats, verbose, "fp64-polish")
+        stats["iters_polish"] = it
+    elif precision == "fp64":
+        X, theta, it = _run_phase(A, X.to(torch.float64).contiguous(), k, ub, torch.float64, tol,
+                                  max_iter, 1e4, True, fresh, stats, verbose, "fp64")
+        # one unlocked Rayleigh-Ritz over the whole block orders and refines
+        X, theta, _ = _run_phase(A, X, k, ub, torch.float64, tol, 20, 1e4, False, fresh, stats,
+                                 verbose, "fp64-final")
+    else:
+        raise ValueError("precision must be 'fp64' or 'mixed'")
+    stats["t_total"] = time.perf_counter() - t0
+    o = torch.argsort(theta)
+    theta, X = theta[o], X[:, o]
+    AX = spmm(A, X[:, :k].contiguous())
+    stats["residual"] = float(_residuals(X[:, :k], theta[:k], AX).max().item() / ub)
+    evals = theta[:k].clone()
+    evecs = X[:, :k].double()
+    evecs = evecs / torch.linalg.vector_norm(evecs, dim=0)
+    return evals, evecs, stats
+
+
+def spectrum_of_scipy(laplacian, n_eigenpairs=None):
+    """compute_spectrum drop-in for a host scipy matrix (geometry.py:66-80)."""
+    N = laplacian.shape[0]
+    k = N if n_eigenpairs is None else min(int(n_eigenpairs), N)
+    A = BSR.from_scipy(laplacian, 1)
+    evals, evecs, _ = chfsi(A, k)
+    return evals.cpu().numpy(), evecs.cpu().numpy() * np.sqrt(N)

@@ -1,0 +1,287 @@
+"""Device GP regression with the spectral connection-Laplacian kernel.
+
+Replaces GPflow's GPR (RVGP/main.py:98-116 subclasses gpflow.models.GPR):
+log marginal likelihood, its gradient (TF autodiff in the reference, a
+hand-written adjoint here) and predict_f.
+
+Two exact forms of the same model y ~ N(0, Phi diag(S) Phi^T + s2 I):
+  * function space: M x M Gram on the matrix cores (rvgp_gemm with the
+    spectral filter as k-scale), blocked Cholesky, triangular solves;
+    optional FP32 Cholesky with FP64 iterative refinement of alpha.
+  * weight space (Woodbury / determinant lemma): k x k system
+    A = s2 S^-1 + Phi^T Phi; G = Phi^T Phi and b = Phi^T y are formed once per
+    training set, so an optimiser step costs O(k^3) and never touches an
+    M x M matrix -- the only feasible form at M = 768k.
+The adjoint forms dK = Phi diag(dS) Phi^T implicitly: only k-vectors
+(diag(Phi^T K^-1 Phi), Phi^T alpha) are needed, never an M x M dK.
+"""
+import math
+
+import numpy as np
+import torch
+
+from . import _lib, dense
+
+TWO_PI = 2.0 * math.pi
+
+
+def softplus(x):
+    return np.logaddexp(0.0, x)
+
+
+def inv_softplus(y):
+    y = np.asarray(y, dtype=np.float64)
+    return y + np.log(-np.expm1(-y))
+
+
+class GPREngine:
+    """Hyperparameters theta = (nu, kappa, sigma_f, s2) = softplus(u) + lower."""
+
+    def __init__(self, Phi, y, evals, n_rows, typ="matern", form="auto", precision="fp64",
+                 lower=(0.0, 0.0, 0.0, 1e-6), fs_max=8192, gram_mode=None,
+                 fs_gram_mode=dense.GEMM_F32):
+        dev = Phi.device
+        self.dev = dev
+        self.M, self.k = Phi.shape
+        self.typ = 0 if typ == "matern" else 1
+        self.n_rows = float(n_rows)
+        self.precision = precision
+        self.form = ("function" if self.M <= fs_max else "weight") if form == "auto" else form
+        self.Phi = Phi.to(torch.float64).contiguous()
+        self.y = y.to(torch.float64).reshape(-1, 1).contiguous()
+        self.evals = evals.to(torch.float64).contiguous()
+        f64 = dict(dtype=torch.float64, device=dev)
+        self.u = torch.zeros(4, **f64)
+        self.lower = torch.as_tensor(np.asarray(lower, np.float64), device=dev)
+        self.theta = torch.zeros(3, **f64)
+        self.s2 = torch.zeros(1, **f64)
+        self.S = torch.empty(self.k, **f64)
+        self.dS = torch.empty(3 * self.k, **f64)
+        self.gS = torch.empty(self.k, **f64)
+        self.out = torch.zeros(9, **f64)
+        self.status = torch.zeros(1, dtype=torch.int32, device=dev)
+        self._cs_ws = torch.empty(_lib.lib().rvgp_colsumsq_ws_bytes(max(self.k, 1)) // 8 + 1, **f64)
+        if self.form == "weight" or precision == "mixed":
+            # k x k normal-equation system (weight space; in the mixed
+            # function-space form it carries log det and diag(Phi^T K^-1 Phi)
+            # in FP64 so that only the alpha solve uses the FP32 factor)
+            if gram_mode is None:
+                gram_mode = dense.GEMM_F64 if precision == "fp64" else dense.GEMM_F32
+            self.gram_mode = gram_mode
+            P = self.Phi if gram_mode == dense.GEMM_F64 else self.Phi.float()
+            G = dense.gemm(P, P, transA=True, mode=gram_mode,
+                           out_dtype=torch.float64 if gram_mode == dense.GEMM_F64 else torch.float32)
+            self.G = G.double().contiguous()
+            self.G = 0.5 * (self.G + self.G.T)
+            self.b = dense.gemm(self.Phi, self.y, transA=True)
+            self.yy = dense.gemm(self.y, self.y, transA=True).reshape(1)
+            self.A = torch.empty((self.k, self.k), **f64)
+            self.w = torch.empty((self.k, 1), **f64)
+            self.Linv = torch.empty((self.k, self.k), **f64)
+            self.eye = torch.eye(self.k, **f64)
+            self.ainv = torch.empty(self.k, **f64)
+        self.fs_gram_mode = fs_gram_mode
+        if self.form == "function":
+            if precision != "mixed":
+                self.K = torch.empty((self.M, self.M), **f64)
+            self.a = torch.empty((self.M, 1), **f64)
+            self.Z = torch.empty_like(self.Phi)
+            self.q = torch.empty(self.k, **f64)
+            self.logdet = torch.empty(1, **f64)
+
+    # ---------------------------------------------------------------- params
+    def set_theta(self, theta):
+        th = np.asarray(theta, np.float64) - self.lower.cpu().numpy()
+        self.u.copy_(torch.as_tensor(inv_softplus(th), device=self.dev))
+
+    def theta_host(self):
+        return softplus(self.u.cpu().numpy()) + self.lower.cpu().numpy()
+
+    def _prologue(self):
+        s = _lib.stream()
+        _lib.call("rvgp_gp_params", _lib.ptr(self.u), _lib.ptr(self.lower), _lib.ptr(self.theta),
+                  _lib.ptr(self.s2), s)
+        _lib.call("rvgp_spectral_filter", self.typ, self.k, _lib.ptr(self.evals),
+                  _lib.ptr(self.theta), self.n_rows, _lib.ptr(self.S), _lib.ptr(self.dS), s)
+
+    def _colsumsq(self, Z, out):
+        _lib.call("rvgp_colsumsq", int(Z.dtype == torch.float64), Z.shape[0], Z.shape[1],
+                  _lib.ptr(Z), Z.stride(0), _lib.ptr(out), _lib.ptr(self._cs_ws),
+                  self._cs_ws.numel() * 8, _lib.stream())
+
+    # ---------------------------------------------------------------- evaluate
+    def evaluate(self):
+        """Enqueue NLML + gradient; results in self.out (device):
+        [nlml, d/d(nu, kappa, sigma_f, s2), d/du(...)]."""
+        self._prologue()
+        s = _lib.stream()
+        self.status.zero_()
+        if self.form == "weight" or self.precision == "mixed":
+            _lib.call("rvgp_ws_form", self.k, _lib.ptr(self.G), self.k, _lib.ptr(self.S),
+                      _lib.ptr(self.s2), _lib.ptr(self.A), self.k, s)
+            _lib.call("rvgp_potrf", 1, self.k, _lib.ptr(self.A), self.k, _lib.ptr(self.status), s)
+            self.w.copy_(self.b)
+            dense.trsm(self.A, self.w)
+            dense.trsm(self.A, self.w, trans=True)
+            self.Linv.copy_(self.eye)
+            dense.trsm(self.A, self.Linv)
+            self._colsumsq(self.Linv, self.ainv)
+        if self.form == "weight":
+            Gw = dense.gemm(self.G, self.w)
+            _lib.call("rvgp_ws_finish", self.k, float(self.M), _lib.ptr(self.A), self.k,
+                      _lib.ptr(self.ainv), _lib.ptr(self.b), _lib.ptr(self.w), _lib.ptr(Gw),
+                      _lib.ptr(self.yy), _lib.ptr(self.S), _lib.ptr(self.dS), _lib.ptr(self.s2),
+                      _lib.ptr(self.u), _lib.ptr(self.gS), _lib.ptr(self.out), s)
+        else:
+            self._factor()
+            # a = K_y^-1 y: L^-1 y first (ya = |L^-1 y|^2), then L^-T
+            self.a.copy_(self.y)
+            self._solve_L(self.a)
+            ya = dense.gemm(self.a, self.a, transA=True).reshape(1)
+            self._solve_LT(self.a)
+            if self.precision == "mixed":
+                self._refine(self.a)
+                ya = dense.gemm(self.y, self.a, transA=True).reshape(1)
+            aa = dense.gemm(self.a, self.a, transA=True).reshape(1)
+            if self.precision == "mixed":
+                _lib.call("rvgp_lemma_terms", self.k, float(self.M), _lib.ptr(self.S),
+                          _lib.ptr(self.s2), _lib.ptr(self.ainv), _lib.ptr(self.A), self.k,
+                          _lib.ptr(self.q), _lib.ptr(self.logdet), s)
+            else:
+                self.Z.copy_(self.Phi)
+                self._solve_L(self.Z)
+                self._colsumsq(self.Z, self.q)
+            p = dense.gemm(self.Phi, self.a, transA=True).reshape(-1)
+            _lib.call("rvgp_fs_finish", self.k, float(self.M), _lib.ptr(self.q), _lib.ptr(p),
+                      _lib.ptr(ya), _lib.ptr(aa), _lib.ptr(self.logdet), _lib.ptr(self.S),
+                      _lib.ptr(self.dS), _lib.ptr(self.s2), _lib.ptr(self.u), _lib.ptr(self.gS),
+                      _lib.ptr(self.out), s)
+        return self.out
+
+    # function-space helpers -------------------------------------------------
+    def _factor(self):
+        """K_y = Phi diag(S) Phi^T + s2 I and its Cholesky factor (f64, or f32
+        in mixed precision)."""
+        if self.precision == "mixed":
+            # FP32 (or BF16-tile) Gram on the matrix cores and FP32 factor: the
+            # preconditioner of the FP64-refined alpha solve only (log det comes
+            # from the FP64 determinant-lemma terms)
+            if not hasattr(self, "K32"):
+                self.Phi32 = self.Phi.float().contiguous()
+                self.K32 = torch.empty((self.M, self.M), dtype=torch.float32, device=self.dev)
+            self.K32.zero_()
+            torch.diagonal(self.K32).copy_(self.s2.float().expand(self.M))
+            dense.gemm(self.Phi32, self.Phi32, self.K32, transB=True, alpha=1.0, beta=1.0,
+                       kscale=self.S, mode=self.fs_gram_mode)
+            _lib.call("rvgp_potrf", 0, self.M, _lib.ptr(self.K32), self.M, _lib.ptr(self.status),
+                      _lib.stream())
+        else:
+            self.K.zero_()
+            torch.diagonal(self.K).copy_(self.s2.expand(self.M))
+            dense.gemm(self.Phi, self.Phi, self.K, transB=True, alpha=1.0, beta=1.0, kscale=self.S)
+            _lib.call("rvgp_potrf", 1, self.M, _lib.ptr(self.K), self.M, _lib.ptr(self.status),
+                      _lib.stream())
+            self.logdet.copy_(dense.chol_logdet_half(self.K))
+
+    def _solve_L(self, B):
+        if self.precision == "mixed":
+            B32 = B.float()
+            dense.trsm(self.K32, B32)
+            B.copy_(B32)
+        else:
+            dense.trsm(self.K, B)
+
+    def _solve_LT(self, B):
+        if self.precision == "mixed":
+            B32 = B.float()
+            dense.trsm(self.K32, B32, trans=True)
+            B.copy_(B32)
+        else:
+            dense.trsm(self.K, B, trans=True)
+
+    def _apply_Ky(self, x):
+        """K_y x = Phi (S * (Phi^T x)) + s2 x, matrix-free in FP64."""
+        t = dense.gemm(self.Phi, x, transA=True)
+        t = t * self.S.reshape(-1, 1)
+        out = dense.gemm(self.Phi, t.contiguous())
+        return out + self.s2 * x
+
+    def _refine(self, x, iters=3):
+        """FP64 iterative refinement of K_y x = y with the FP32 factor."""
+        for _ in range(iters):
+            r = self.y - self._apply_Ky(x)
+            self._solve_L(r)
+            self._solve_LT(r)
+            x += r
+
+    # ---------------------------------------------------------------- host API
+    def nlml_grad_u(self, u):
+        self.u.copy_(torch.as_tensor(np.asarray(u, np.float64), device=self.dev))
+        out = self.evaluate().cpu().numpy()
+        if int(self.status.item()) != 0:
+            raise dense.CholeskyError("Cholesky decomposition was not successful: the GP system "
+                                      "matrix is not positive definite")
+        return float(out[0]), out[5:9].copy(), out
+
+    def predict(self, Phi_star):
+        """Mean (m*,) and marginal variance (m*,) of f at rows Phi_star,
+        as GPflow's GPR.predict_f(full_cov=False)."""
+        self.evaluate()
+        Ps = Phi_star.to(torch.float64).contiguous()
+        m = Ps.shape[0]
+        if self.form == "weight" or self.precision == "mixed":
+            # mean = Phi* w, var = s2 |L_A^-1 phi*|^2 from the FP64 k x k system
+            mean = dense.gemm(Ps, self.w).reshape(-1)
+            Zt = Ps.T.contiguous()
+            dense.trsm(self.A, Zt)
+            v = torch.empty(m, dtype=torch.float64, device=self.dev)
+            cs_ws = torch.empty(_lib.lib().rvgp_colsumsq_ws_bytes(m) // 8 + 1, dtype=torch.float64,
+                                device=self.dev)
+            _lib.call("rvgp_colsumsq", 1, self.k, m, _lib.ptr(Zt), m, _lib.ptr(v), _lib.ptr(cs_ws),
+                      cs_ws.numel() * 8, _lib.stream())
+            var = self.s2 * v
+        else:
+            p = dense.gemm(self.Phi, self.a, transA=True)
+            mean = dense.gemm(Ps, (p * self.S.reshape(-1, 1)).contiguous()).reshape(-1)
+            Kms = dense.gemm(self.Phi, Ps, transB=True, kscale=self.S)  # (M, m*)
+            self._solve_L(Kms)
+            cs_ws = torch.empty(_lib.lib().rvgp_colsumsq_ws_bytes(m) // 8 + 1, dtype=torch.float64,
+                                device=self.dev)
+            v = torch.empty(m, dtype=torch.float64, device=self.dev)
+            _lib.call("rvgp_colsumsq", 1, self.M, m, _lib.ptr(Kms), m, _lib.ptr(v), _lib.ptr(cs_ws),
+                      cs_ws.numel() * 8, _lib.stream())
+            Zs = (Ps.T * torch.sqrt(self.S).reshape(-1, 1)).contiguous()
+            d = torch.empty(m, dtype=torch.float64, device=self.dev)
+            _lib.call("rvgp_colsumsq", 1, self.k, m, _lib.ptr(Zs), m, _lib.ptr(d), _lib.ptr(cs_ws),
+                      cs_ws.numel() * 8, _lib.stream())
+            var = d - v
+        return mean, var
+
+    # ---------------------------------------------------------------- optimisers
+    def fit_lbfgs(self, maxiter=1000, disp=False):
+        import scipy.optimize
+        u0 = self.u.cpu().numpy()
+
+        def f(u):
+            val, g, _ = self.nlml_grad_u(u)
+            return val, g
+
+        res = scipy.optimize.minimize(f, u0, jac=True, method="L-BFGS-B",
+                                      options={"disp": disp, "maxiter": maxiter})
+        self.u.copy_(torch.as_tensor(res.x, device=self.dev))
+        return res
+
+    def adam_setup(self, trainable=(1, 1, 1, 1)):
+        f64 = dict(dtype=torch.float64, device=self.dev)
+        self.m1 = torch.zeros(4, **f64)
+        self.m2 = torch.zeros(4, **f64)
+        self.t = torch.zeros(1, **f64)
+        self.trainable = torch.as_tensor(np.asarray(trainable, np.int32), device=self.dev)
+
+    def adam_step(self, lr=0.05, b1=0.9, b2=0.999, eps=1e-8):
+        """One device-resident optimiser step: NLML, adjoint, Adam update.
+        No host synchronisation, so it can be captured in a HIP graph."""
+        self.evaluate()
+        _lib.call("rvgp_adam_step", _lib.ptr(self.u), _lib.ptr(self.out[5:]), _lib.ptr(self.m1),
+                  _lib.ptr(self.m2), _lib.ptr(self.t), _lib.ptr(self.trainable), float(lr),
+                  float(b1), float(b2), float(eps), _lib.stream())

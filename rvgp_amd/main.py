@@ -1,0 +1,153 @@
+"""Drop-in for RVGP/main.py: ``RVGP.fit`` (train_gp) and the fitted model's
+``transform``.
+
+With ``reference_semantics=True`` (the default) the reference's behaviour is
+kept, quirks included: the 80/20 re-split of ``train_ind`` with
+sklearn's train_test_split rule (main.py:40-45), the noise variance starting
+at 1.0 whatever ``noise_variance`` says (main.py:99-100 passes None to
+GPflow), the kernel parameters created before
+``set_default_positive_minimum`` (main.py:30 vs 52), AttributeError for
+``kernel_variance`` / ``kernel_lengthscale`` (ManifoldKernel has no
+``variance`` / ``lengthscales``), L-BFGS-B with ``maxiter=epochs``, and the
+int / float / other dispatch of ``transform`` (main.py:104-109).
+``reference_semantics=False`` honours ``noise_variance`` and skips the
+re-split.  All numerical work (NLML, adjoint, predictions) runs on the GPU
+through gp.GPREngine.
+"""
+import numpy as np
+import torch
+
+from . import _lib
+from .gp import GPREngine
+from .kernels import ManifoldKernel, set_default_positive_minimum
+
+GAUSSIAN_VARIANCE_LOWER_BOUND = 1e-6  # gpflow.likelihoods.Gaussian default
+
+
+def train_test_split_indices(n, test_size=0.2, seed=0):
+    """Index rule of sklearn.model_selection.train_test_split(random_state=seed)
+    for a float test_size: n_test = ceil(test_size * n), permutation from
+    RandomState(seed), test = perm[:n_test], train = perm[n_test:]."""
+    n_test = int(np.ceil(test_size * n))
+    perm = np.random.RandomState(seed).permutation(n)
+    return perm[n_test:], perm[:n_test]
+
+
+class manifold_GPR:
+    """The fitted model (main.py:98-116)."""
+
+    def __init__(self, data, kernel, train_nodes, engine, reference_semantics=True):
+        self.data = data
+        self.kernel = kernel
+        self.train_nodes = train_nodes
+        self.engine = engine
+        self.reference_semantics = reference_semantics
+
+    @property
+    def hyperparameters(self):
+        th = self.engine.theta_host()
+        return {"nu": th[0], "kappa": th[1], "sigma_f": th[2], "noise_variance": th[3]}
+
+    def training_loss(self):
+        u = self.engine.u.cpu().numpy()
+        return self.engine.nlml_grad_u(u)[0]
+
+    def predict_f(self, Xnew):
+        """Mean and marginal variance at spectral rows Xnew ((m, k) array or
+        device tensor), as GPflow's predict_f(full_cov=False)."""
+        dev = self.engine.dev
+        Xd = Xnew if isinstance(Xnew, torch.Tensor) else torch.as_tensor(
+            np.asarray(Xnew, np.float64), device=dev)
+        mean, var = self.engine.predict(Xd)
+        return mean.reshape(-1, 1), var.reshape(-1, 1)
+
+    def transform(self, data, test_ind):
+        """Predictive mean and variance at test nodes (main.py:102-116)."""
+        first = test_ind[0]
+        ints = (int,) if self.reference_semantics else (int, np.integer)
+        floats = (float,) if self.reference_semantics else (float, np.floating)
+        if isinstance(first, ints):
+            idx = np.asarray(test_ind)
+            if idx.dtype == bool:  # a Python bool list indexes as a mask in numpy
+                idx = np.nonzero(idx)[0]
+            nodes = torch.as_tensor(idx.astype(np.int32), device=self.engine.dev)
+            test_x = data.phi_rows(nodes)
+        elif isinstance(first, floats):
+            test_x = np.asarray(test_ind, dtype=np.float64).reshape(-1, self.engine.k)
+        else:  # np.int64 / np.bool_ match neither branch of main.py:104-109
+            raise UnboundLocalError("local variable 'test_x' referenced before assignment")
+        mean, var = self.predict_f(test_x)
+        mean = mean.cpu().numpy().reshape(len(test_ind), -1)
+        var = var.cpu().numpy().reshape(len(test_ind), -1)
+        return mean, var
+
+
+def train_gp(data, train_ind=None, n_inducing_points=None, test_size=0.2, kernel=None,
+             noise_variance=0.001, kernel_lengthscale=None, kernel_variance=None, epochs=1000,
+             positivity_constraint=1e-2, seed=0, *, reference_semantics=True, optimizer="lbfgs",
+             lr=0.05, form="auto", precision="fp64", verbose=True):
+    """Fit the vector-field GP (main.py:11-84)."""
+    if train_ind is None:
+        train_ind = np.arange(data.n)
+    train_ind = np.asarray(train_ind)
+    if train_ind.dtype == bool:
+        train_ind = np.nonzero(train_ind)[0]
+    if kernel is None:
+        kernel = ManifoldKernel(data, nu=3 / 2, kappa=5, typ='matern', sigma_f=1.)
+    elif kernel == 'rbf':
+        raise NotImplementedError("kernel='rbf' (main.py:33-37) is not on the GPU path")
+    else:
+        raise UnboundLocalError("local variable 'input' referenced before assignment")
+
+    if reference_semantics:
+        tr_pos, te_pos = train_test_split_indices(len(train_ind), test_size, seed)
+        nodes_tr, nodes_te = train_ind[tr_pos], train_ind[te_pos]
+    else:
+        nodes_tr, nodes_te = train_ind, train_ind[:0]
+
+    set_default_positive_minimum(positivity_constraint)
+    if n_inducing_points is not None:
+        raise NotImplementedError("SGPR inducing points (main.py:59-67) are not on the GPU path")
+
+    if kernel_variance is not None:
+        raise AttributeError("'ManifoldKernel' object has no attribute 'variance'")
+    if kernel_lengthscale is not None:
+        raise AttributeError("'ManifoldKernel' object has no attribute 'lengthscales'")
+
+    dev = _lib.device()
+    nd_tr = torch.as_tensor(nodes_tr.astype(np.int32), device=dev)
+    Phi = data.phi_rows(nd_tr)
+    V = data.vectors_device()
+    if V is None:
+        raise TypeError("data.vectors is None: nothing to fit")
+    y = V[torch.as_tensor(nodes_tr, device=dev).long()].reshape(-1)
+    noise0 = 1.0 if reference_semantics else float(noise_variance)
+    lower = (kernel.nu.lower, kernel.kappa.lower, kernel.sigma_f.lower,
+             GAUSSIAN_VARIANCE_LOWER_BOUND)
+    eng = GPREngine(Phi, y, data.spectrum_device(), kernel.num_verticies, kernel.typ, form=form,
+                    precision=precision, lower=lower)
+    eng.set_theta([float(kernel.nu), float(kernel.kappa), float(kernel.sigma_f), noise0])
+    if optimizer == "lbfgs":
+        eng.fit_lbfgs(maxiter=epochs, disp=verbose)
+    elif optimizer == "adam":
+        eng.adam_setup()
+        for _ in range(int(epochs)):
+            eng.adam_step(lr=lr)
+        torch.cuda.synchronize()
+    else:
+        raise ValueError("optimizer must be 'lbfgs' or 'adam'")
+    th = eng.theta_host()
+    kernel.nu.assign(th[0])
+    kernel.kappa.assign(th[1])
+    kernel.sigma_f.assign(th[2])
+    model = manifold_GPR(data, kernel, nodes_tr, eng, reference_semantics)
+
+    if len(nodes_te):
+        mean, _ = model.predict_f(data.phi_rows(torch.as_tensor(nodes_te.astype(np.int32), device=dev)))
+        out_test = V[torch.as_tensor(nodes_te, device=dev).long()].reshape(-1, 1)
+        # np.linalg.norm(out_test - out_pred, axis=1).mean() on (rows, 1) arrays
+        err = (out_test - mean).abs().mean().item()
+        model.heldout_error = err
+        if verbose:
+            print("Relative l2 error is {}".format(err))
+    return model

@@ -1,0 +1,83 @@
+"""CPU tests of the C ABI and host logic (no GPU compute calls)."""
+import os
+import re
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    src = open(os.path.join(ROOT, "include", "rvgp.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(rvgp_\w+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol():
+    from rvgp_amd import _lib
+    assert os.path.exists(_lib.LIB_PATH), "run __graft_entry__.build() first"
+    L = _lib.load_symbols_only()
+    names = declared_symbols()
+    assert len(names) >= 25
+    for name in names:
+        assert hasattr(L, name), name
+        assert name in _lib.SIGNATURES, f"{name} missing from the ctypes table"
+    assert L.rvgp_version() >= 1
+
+
+def test_ctypes_table_has_no_undeclared_entries():
+    from rvgp_amd import _lib
+    assert set(_lib.SIGNATURES) <= set(declared_symbols())
+
+
+def test_arg_validation_reports_value_errors_without_a_gpu():
+    """Argument checks run before any device work: null pointers are fine."""
+    from rvgp_amd import _lib
+    L = _lib.load_symbols_only()
+    rc = L.rvgp_knn(None, 10, 3, 10, None, None)  # k >= n
+    assert rc == _lib.RVGP_EINVAL
+    assert b"0 < k < n" in L.rvgp_last_error()
+    rc = L.rvgp_bsr_spmm_f64(10, 4, None, None, None, None, 1, 1, 1.0, 0.0, None, 1, 0.0, None, 1,
+                             None, 1, 0.0, None)
+    assert rc == _lib.RVGP_EINVAL
+    rc = L.rvgp_gemm(7, 0, 0, 1, 1, 1, 1.0, None, 1, None, 1, None, 0.0, None, 1, 1, None, 0, None)
+    assert rc == _lib.RVGP_EINVAL
+    rc = L.rvgp_gauges(None, 10, 65, None, 10, 1, 1, 1, None, None, None, None)
+    assert rc == _lib.RVGP_EINVAL
+
+
+def test_workspace_queries():
+    from rvgp_amd import _lib
+    L = _lib.load_symbols_only()
+    assert L.rvgp_graph_symmetrize_ws_bytes(100, 10) >= 4 * 100 * 21
+    assert L.rvgp_geo_nbh_ws_bytes(2, 100) == 2 * L.rvgp_geo_nbh_ws_bytes(1, 100)
+    assert L.rvgp_colsumsq_ws_bytes(10) >= 8 * 10
+
+
+def test_host_split_rule_and_softplus():
+    from rvgp_amd.main import train_test_split_indices
+    from rvgp_amd.gp import inv_softplus, softplus
+    from sklearn.model_selection import train_test_split
+    tr, te = train_test_split(np.arange(101), test_size=0.2, random_state=3)
+    tr2, te2 = train_test_split_indices(101, 0.2, 3)
+    np.testing.assert_array_equal(tr, tr2)
+    np.testing.assert_array_equal(te, te2)
+    x = np.array([1e-6, 0.3, 5.0, 40.0])
+    np.testing.assert_allclose(softplus(inv_softplus(x)), x, rtol=1e-12)
+
+
+def test_chebyshev_heat_coefficients():
+    """The truncated Chebyshev series of exp(-t x) on [0, ub] is exact to
+    ~1e-16 at sample points (host-side coefficient logic)."""
+    from rvgp_amd.smoothing import cheb_coeffs
+    for t, ub in ((0.5, 10.0), (100.0, 20.0), (500.0, 45.0)):
+        c = cheb_coeffs(t, ub)
+        x = np.linspace(0, ub, 41)
+        y = np.polynomial.chebyshev.chebval(2 * x / ub - 1, c)
+        np.testing.assert_allclose(y, np.exp(-t * x), atol=1e-14)
+
+
+def test_package_imports_without_gpu():
+    import rvgp_amd
+    from rvgp_amd import geometry, dataclass, main, kernels, smoothing, eigsolve  # noqa: F401
+    assert callable(rvgp_amd.create_data_object) and callable(rvgp_amd.fit)

@@ -1,0 +1,96 @@
+"""Matrix-core GEMM, Cholesky and triangular solves vs torch / numpy FP64."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _t(x, dt):
+    import torch
+    return torch.as_tensor(np.ascontiguousarray(x), device="cuda", dtype=dt)
+
+
+@pytest.mark.parametrize("M,N,K", [(70, 33, 45), (64, 64, 16), (1, 5, 3), (130, 200, 257)])
+@pytest.mark.parametrize("tA", [False, True])
+@pytest.mark.parametrize("tB", [False, True])
+def test_gemm_modes(M, N, K, tA, tB):
+    import torch
+    from rvgp_amd import dense
+    rng = np.random.default_rng(M * 7 + N + K)
+    A = rng.normal(size=(K, M) if tA else (M, K))
+    B = rng.normal(size=(N, K) if tB else (K, N))
+    C0 = rng.normal(size=(M, N))
+    s = rng.uniform(0.5, 2.0, size=K)
+    opA = A.T if tA else A
+    opB = B.T if tB else B
+    ref = 0.75 * (opA * s) @ opB - 0.5 * C0
+    # f64
+    C = _t(C0, torch.float64)
+    dense.gemm(_t(A, torch.float64), _t(B, torch.float64), C, tA, tB, 0.75, -0.5, _t(s, torch.float64))
+    np.testing.assert_allclose(C.cpu().numpy(), ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
+    # f32 in, f64 accumulate: exact products of the f32-rounded inputs
+    A32, B32 = A.astype(np.float32).astype(np.float64), B.astype(np.float32).astype(np.float64)
+    ref32 = 0.75 * ((A32.T if tA else A32) * s) @ (B32.T if tB else B32) - 0.5 * C0
+    C = _t(C0, torch.float64)
+    dense.gemm(_t(A, torch.float32), _t(B, torch.float32), C, tA, tB, 0.75, -0.5, _t(s, torch.float64))
+    np.testing.assert_allclose(C.cpu().numpy(), ref32, rtol=1e-10, atol=1e-10 * np.abs(ref).max())
+    # f32
+    C = _t(C0, torch.float32)
+    dense.gemm(_t(A, torch.float32), _t(B, torch.float32), C, tA, tB, 0.75, -0.5, _t(s, torch.float64))
+    assert np.abs(C.cpu().numpy() - ref).max() < 1e-5 * np.abs(ref).max() * np.sqrt(K)
+    # bf16 tiles, f32 accumulation
+    C = _t(C0, torch.float32)
+    dense.gemm(_t(A, torch.float32), _t(B, torch.float32), C, tA, tB, 0.75, -0.5,
+               _t(s, torch.float64), mode=dense.GEMM_BF16)
+    assert np.abs(C.cpu().numpy() - ref).max() < 2e-2 * np.abs(ref).max()
+
+
+def test_gemm_split_k_tall_skinny():
+    import torch
+    from rvgp_amd import dense
+    rng = np.random.default_rng(0)
+    X = rng.normal(size=(40000, 90))
+    Y = rng.normal(size=(40000, 70))
+    ref = X.T @ Y
+    for split in (None, 1, 7):
+        C = dense.gemm(_t(X, torch.float64), _t(Y, torch.float64), transA=True, split_k=split)
+        np.testing.assert_allclose(C.cpu().numpy(), ref, rtol=1e-11, atol=1e-11 * np.abs(ref).max())
+    C1 = dense.gemm(_t(X, torch.float64), _t(Y, torch.float64), transA=True, split_k=9)
+    C2 = dense.gemm(_t(X, torch.float64), _t(Y, torch.float64), transA=True, split_k=9)
+    assert torch.equal(C1, C2)  # deterministic split-K reduction
+
+
+@pytest.mark.parametrize("n", [1, 5, 63, 64, 65, 200, 333])
+@pytest.mark.parametrize("f64", [True, False])
+def test_potrf_trsm_logdet(n, f64):
+    import torch
+    from rvgp_amd import dense
+    rng = np.random.default_rng(n)
+    G = rng.normal(size=(n, n + 3))
+    A = G @ G.T + n * np.eye(n)
+    dt = torch.float64 if f64 else torch.float32
+    tol = 1e-11 if f64 else 2e-4
+    L = _t(A, dt)
+    dense.potrf(L)
+    Lh = np.tril(L.cpu().double().numpy())
+    Lr = np.linalg.cholesky(A)
+    np.testing.assert_allclose(Lh, Lr, rtol=0, atol=tol * np.abs(Lr).max())
+    Bm = rng.normal(size=(n, 7))
+    X = _t(Bm, dt)
+    dense.trsm(L, X)
+    np.testing.assert_allclose(X.cpu().double().numpy(), np.linalg.solve(Lr, Bm), rtol=0,
+                               atol=tol * 10 * np.abs(np.linalg.solve(Lr, Bm)).max())
+    dense.trsm(L, X, trans=True)
+    sol = np.linalg.solve(A, Bm)
+    np.testing.assert_allclose(X.cpu().double().numpy(), sol, rtol=0, atol=tol * 10 * np.abs(sol).max())
+    ld = dense.chol_logdet_half(L).item()
+    assert abs(2 * ld - np.linalg.slogdet(A)[1]) < (1e-10 if f64 else 1e-3) * max(1.0, abs(ld))
+
+
+def test_potrf_not_spd():
+    import torch
+    from rvgp_amd import dense
+    A = np.eye(70)
+    A[40, 40] = -1.0
+    with pytest.raises(dense.CholeskyError):
+        dense.potrf(_t(A, torch.float64))

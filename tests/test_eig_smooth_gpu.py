@@ -1,0 +1,185 @@
+"""Eigensolver (a8) and heat-kernel smoothing (a11) on the GPU vs FP64 oracles.
+
+Eigenvalues: relative error <= 1e-6 (absolute floor 1e-9 * ub for ~0
+eigenvalues); eigenspaces: largest principal angle <= 1e-4 rad over clusters
+delimited by spectral gaps (pairs / multiplets are compared as subspaces,
+never vector by vector).
+"""
+import numpy as np
+import pytest
+import scipy.linalg
+import scipy.sparse as sp
+
+from oracle import cpu
+from tests.util import sphere, torus
+
+pytestmark = pytest.mark.gpu
+
+
+def _bsr_from_blocks(rp, c, blocks, b):
+    import torch
+    from rvgp_amd.sparse import BSR
+    return BSR(torch.as_tensor(rp, device="cuda"), torch.as_tensor(c, device="cuda"),
+               torch.as_tensor(np.ascontiguousarray(blocks).reshape(-1), device="cuda"), None, b)
+
+
+def ring_connection_laplacian(n, theta):
+    """Cycle graph with self loops; transport R(theta/n) along i -> i+1."""
+    rp = [0]
+    cols, blocks = [], []
+    phi = theta / n
+    Rf = np.array([[np.cos(phi), -np.sin(phi)], [np.sin(phi), np.cos(phi)]])
+    for i in range(n):
+        nb = sorted({(i - 1) % n, i, (i + 1) % n})
+        for j in nb:
+            cols.append(j)
+            if j == i:
+                blocks.append(2.0 * np.eye(2))
+            elif j == (i + 1) % n:
+                blocks.append(-Rf)
+            else:
+                blocks.append(-Rf.T)
+        rp.append(len(cols))
+    return np.array(rp, np.int32), np.array(cols, np.int32), np.array(blocks)
+
+
+def cluster_angles(w_ref, V_ref, V, k, rel_gap=1e-5):
+    """Max principal angle over gap-delimited clusters fully inside [0, k)."""
+    scale = max(abs(w_ref[k - 1]), 1e-12)
+    worst = 0.0
+    start = 0
+    for j in range(1, k + 1):
+        if j == k or (w_ref[j] - w_ref[j - 1]) > rel_gap * scale:
+            if j == k and (k < len(w_ref)) and (w_ref[k] - w_ref[k - 1]) <= rel_gap * scale:
+                break  # cluster straddles the cut at k
+            ang = scipy.linalg.subspace_angles(V_ref[:, start:j], V[:, start:j]).max()
+            worst = max(worst, ang)
+            start = j
+    return worst
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_ring_closed_forms(precision):
+    from rvgp_amd.eigsolve import chfsi
+    n, theta, k = 300, 1.3, 24
+    rp, c, blocks = ring_connection_laplacian(n, theta)
+    A = _bsr_from_blocks(rp, c, blocks, 2)
+    w, V, st = chfsi(A, k, precision=precision)
+    # real 2n x 2n form: each 2 - 2cos((2 pi j + theta)/n), j in Z, appears twice
+    j = np.arange(n)
+    ref = 2 - 2 * np.cos((2 * np.pi * j + theta) / n)
+    ref = np.sort(np.concatenate([ref, ref]))[:k]
+    w = w.cpu().numpy()
+    np.testing.assert_allclose(w, ref, rtol=1e-6, atol=1e-9 * 4)
+    # scalar cycle: 2 - 2 cos(2 pi j / n)
+    rp1 = rp
+    A1 = _bsr_from_blocks(rp1, c, np.array([2.0 if (i == cc) else -1.0 for i, cc in
+                                            zip(np.repeat(np.arange(n), np.diff(rp1)), c)]), 1)
+    w1, V1, _ = chfsi(A1, 9, precision=precision)
+    ref1 = np.sort(2 - 2 * np.cos(2 * np.pi * np.arange(n) / n))[:9]
+    np.testing.assert_allclose(w1.cpu().numpy(), ref1, rtol=1e-6, atol=1e-9 * 4)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_sphere_connection_laplacian_vs_eigh(precision):
+    """C1-style operator (S^2, k=10, d=2) at n=1200: eigenvalues and
+    cluster subspaces vs numpy.linalg.eigh of the dense FP64 matrix."""
+    import torch
+    from rvgp_amd.eigsolve import chfsi
+    X = sphere(1200, 0)
+    rp, c = cpu.sym_csr(cpu.knn(X, 10))
+    T, S = cpu.tangent_frames(X, rp, c, 3, 15)
+    R = cpu.connections(np.ascontiguousarray(T[:, :, :2]), rp, c)
+    blocks = cpu.lc_blocks(rp, c, R)
+    A = _bsr_from_blocks(rp, c, blocks, 2)
+    Lc = cpu.connection_laplacian(rp, c, R)
+    w_ref, V_ref = np.linalg.eigh(Lc.toarray())
+    k = 50
+    w, V, st = chfsi(A, k, precision=precision)
+    w, V = w.cpu().numpy(), V.cpu().numpy()
+    ub = A.gershgorin_bound()
+    err = np.abs(w - w_ref[:k]) / np.maximum(np.abs(w_ref[:k]), 1e-3 * ub)
+    assert err.max() < 1e-6, (err.max(), st)
+    assert cluster_angles(w_ref, V_ref, V, k) < 1e-4
+    # scalar Laplacian of the same graph
+    L = cpu.laplacian(rp, c)
+    rows = np.repeat(np.arange(1200), np.diff(rp))
+    A1 = _bsr_from_blocks(rp, c, np.asarray(L[rows, c]).ravel(), 1)
+    w1, V1, _ = chfsi(A1, k, precision=precision)
+    wr1, Vr1 = np.linalg.eigh(L.toarray())
+    w1 = w1.cpu().numpy()
+    assert abs(w1[0]) < 1e-9 * ub
+    np.testing.assert_allclose(w1[1:], wr1[1:k], rtol=1e-6)
+    assert cluster_angles(wr1, Vr1, V1.cpu().numpy(), k) < 1e-4
+
+
+def test_eig_rejects_k_ge_n_minus_1():
+    from rvgp_amd.eigsolve import chfsi
+    rp, c, blocks = ring_connection_laplacian(10, 0.5)
+    A = _bsr_from_blocks(rp, c, blocks, 2)
+    with pytest.raises(TypeError):
+        chfsi(A, 19)
+
+
+# ---------------------------------------------------------------- smoothing
+@pytest.mark.parametrize("t", [0.3, 5.0, 100.0])
+def test_expm_multiply_vs_scipy_expm(t):
+    import torch
+    from rvgp_amd.smoothing import expm_multiply
+    X = sphere(250, 3)
+    rp, c = cpu.sym_csr(cpu.knn(X, 8))
+    T, S = cpu.tangent_frames(X, rp, c, 3, 12)
+    R = cpu.connections(np.ascontiguousarray(T[:, :, :2]), rp, c)
+    A = _bsr_from_blocks(rp, c, cpu.lc_blocks(rp, c, R), 2)
+    Lc = cpu.connection_laplacian(rp, c, R)
+    x = np.random.default_rng(1).normal(size=(500, 3))
+    ref = scipy.linalg.expm(-t * Lc.toarray()) @ x
+    out = expm_multiply(A, torch.as_tensor(x, device="cuda"), t).cpu().numpy()
+    assert np.abs(out - ref).max() < 1e-12 * max(1.0, np.abs(x).max()) * 50
+
+
+def test_heat_kernel_preserves_constants():
+    """exp(-tL) 1 = 1 for a graph Laplacian (row sums zero), any t."""
+    import torch
+    from rvgp_amd.smoothing import expm_multiply
+    X = torus(5000, 2)
+    rp, c = cpu.sym_csr(cpu.knn(X, 15))
+    L = cpu.laplacian(rp, c)
+    rows = np.repeat(np.arange(5000), np.diff(rp))
+    A = _bsr_from_blocks(rp, c, np.asarray(L[rows, c]).ravel(), 1)
+    one = torch.ones(5000, dtype=torch.float64, device="cuda")
+    for t in (1.0, 50.0, 500.0):
+        out = expm_multiply(A, one, t).cpu().numpy()
+        assert np.abs(out - 1.0).max() < 1e-11
+
+
+def test_vector_diffusion_vs_oracle():
+    """smoothing.vector_diffusion (normalised) vs the dense-expm oracle."""
+    import torch
+    from rvgp_amd.smoothing import vector_diffusion_device
+    X = sphere(300, 4)
+    rp, c = cpu.sym_csr(cpu.knn(X, 10))
+    T, S = cpu.tangent_frames(X, rp, c, 3, 15)
+    T2 = np.ascontiguousarray(T[:, :, :2])
+    R = cpu.connections(T2, rp, c)
+    Lc = cpu.connection_laplacian(rp, c, R)
+    L = cpu.laplacian(rp, c)
+    A = _bsr_from_blocks(rp, c, cpu.lc_blocks(rp, c, R), 2)
+    rows = np.repeat(np.arange(300), np.diff(rp))
+    A1 = _bsr_from_blocks(rp, c, np.asarray(L[rows, c]).ravel(), 1)
+    v = np.random.default_rng(5).normal(size=(300, 3))
+    cl = cpu.express_in_local_frame(v, T2)
+    for t in (1.0, 100.0):
+        ref = cpu.vector_diffusion(cl, t, Lc, L, "expm")
+        out = vector_diffusion_device(torch.as_tensor(cl, device="cuda"), t, A, A1).cpu().numpy()
+        # the renormalisation divides by |exp(-t Lc) c|, which has decayed by
+        # amp = |c| / |exp(-t Lc) c|; FP64 rounding is amplified by that factor
+        raw = cpu.heat(Lc, t, cl.reshape(-1, 1), "eigh")
+        amp = np.linalg.norm(cl) / np.linalg.norm(raw)
+        assert np.abs(out - ref).max() < 1e-13 * amp * np.abs(ref).max()
+    # batched fields = one multi-column pass
+    cl2 = np.stack([cl, 2 * cl[::-1].copy()])
+    outb = vector_diffusion_device(torch.as_tensor(cl2, device="cuda"), 3.0, A, A1).cpu().numpy()
+    for f in range(2):
+        ref = cpu.vector_diffusion(cl2[f], 3.0, Lc, L, "expm")
+        assert np.abs(outb[f] - ref).max() < 1e-10 * np.abs(ref).max()

@@ -1,0 +1,150 @@
+"""GP layer on the GPU (a12, a14, a16) vs the FP64 textbook oracle.
+
+The spectral filter, NLML, its adjoint gradient and the predictive
+mean / variance are checked at fixed hyperparameters on the same Phi, y;
+function-space and weight-space forms must agree with each other; the
+gradient is also checked against central finite differences of the GPU NLML.
+"""
+import numpy as np
+import pytest
+
+from oracle import cpu
+
+pytestmark = pytest.mark.gpu
+
+
+def problem(M_nodes=300, D=3, k=40, seed=0):
+    rng = np.random.default_rng(seed)
+    lam = np.sort(rng.uniform(0.0, 5.0, size=k))
+    lam[0] = 0.0
+    Phi = rng.normal(size=(M_nodes * D, k))
+    y = rng.normal(size=(M_nodes * D, 1))
+    return Phi, y, lam
+
+
+def engine(Phi, y, lam, n_rows, form, precision="fp64", theta=(1.5, 5.0, 1.0, 0.3)):
+    import torch
+    from rvgp_amd.gp import GPREngine
+    e = GPREngine(torch.as_tensor(Phi, device="cuda"), torch.as_tensor(y, device="cuda"),
+                  torch.as_tensor(lam, device="cuda"), n_rows, form=form, precision=precision)
+    e.set_theta(theta)
+    return e
+
+
+@pytest.mark.parametrize("typ", ["matern", "se"])
+def test_spectral_filter(typ):
+    import torch
+    from rvgp_amd import _lib
+    lam = np.linspace(0, 7, 60)
+    th = (1.3, 2.7, 0.8)
+    S = torch.empty(60, dtype=torch.float64, device="cuda")
+    dS = torch.empty(180, dtype=torch.float64, device="cuda")
+    thd = torch.tensor(th, dtype=torch.float64, device="cuda")
+    _lib.call("rvgp_spectral_filter", 0 if typ == "matern" else 1, 60,
+              _lib.ptr(torch.as_tensor(lam, device="cuda")), _lib.ptr(thd), 1234.0, _lib.ptr(S),
+              _lib.ptr(dS), _lib.stream())
+    S_ref, dS_ref = cpu.spectral_filter(lam, *th, 1234.0, typ)
+    np.testing.assert_allclose(S.cpu().numpy(), S_ref, rtol=1e-13)
+    np.testing.assert_allclose(dS.cpu().numpy().reshape(3, 60), dS_ref, rtol=1e-11, atol=1e-13)
+    assert abs(S.sum().item() - 1234.0 * th[2]) < 1e-9  # normalisation (kernels.py:46)
+
+
+@pytest.mark.parametrize("form", ["function", "weight"])
+def test_nlml_and_gradient_vs_oracle(form):
+    Phi, y, lam = problem()
+    n_rows = 2000.0
+    th = (1.5, 5.0, 1.0, 0.3)
+    e = engine(Phi, y, lam, n_rows, form, theta=th)
+    out = e.evaluate().cpu().numpy()
+    S, dS = cpu.spectral_filter(lam, th[0], th[1], th[2], n_rows)
+    ref = cpu.gpr_nlml(Phi, y, S, th[3])
+    gS, gn = cpu.gpr_grad(Phi, y, S, th[3])
+    gref = np.array([dS[0] @ gS, dS[1] @ gS, dS[2] @ gS, gn])
+    assert abs(out[0] - ref) < 1e-10 * abs(ref)
+    np.testing.assert_allclose(out[1:5], gref, rtol=1e-8, atol=1e-9 * np.abs(gref).max())
+
+
+def test_function_and_weight_space_agree():
+    Phi, y, lam = problem(M_nodes=400, k=60, seed=3)
+    a = engine(Phi, y, lam, 5000.0, "function")
+    b = engine(Phi, y, lam, 5000.0, "weight")
+    oa, ob = a.evaluate().cpu().numpy(), b.evaluate().cpu().numpy()
+    np.testing.assert_allclose(oa, ob, rtol=1e-9)
+    Ps = np.random.default_rng(9).normal(size=(50, 60))
+    import torch
+    ma, va = a.predict(torch.as_tensor(Ps, device="cuda"))
+    mb, vb = b.predict(torch.as_tensor(Ps, device="cuda"))
+    # FP64 on both sides; K_y has condition ~1e5-1e6 here, so the two
+    # algebraically equal forms agree to ~1e-8 relative, not to 1e-15
+    ma, mb = ma.cpu().numpy(), mb.cpu().numpy()
+    assert np.linalg.norm(ma - mb) / np.linalg.norm(ma) < 1e-7
+    np.testing.assert_allclose(va.cpu().numpy(), vb.cpu().numpy(), rtol=1e-7, atol=1e-12)
+
+
+@pytest.mark.parametrize("form", ["function", "weight"])
+def test_gradient_vs_finite_differences(form):
+    Phi, y, lam = problem(M_nodes=200, k=30, seed=5)
+    e = engine(Phi, y, lam, 900.0, form)
+    u0 = e.u.cpu().numpy().copy()
+    _, g, _ = e.nlml_grad_u(u0)
+    # central differences with h = 1e-4: truncation ~1e-8, rounding noise of an
+    # O(1e3) NLML ~1e-12 relative -> ~1e-5 absolute; the exact adjoint is
+    # pinned to 1e-8 against the oracle's closed form elsewhere
+    h = 1e-4
+    for p in range(4):
+        up, um = u0.copy(), u0.copy()
+        up[p] += h
+        um[p] -= h
+        fd = (e.nlml_grad_u(up)[0] - e.nlml_grad_u(um)[0]) / (2 * h)
+        assert abs(fd - g[p]) < 2e-5 * max(1.0, abs(fd)), (p, fd, g[p])
+
+
+def test_predict_vs_oracle():
+    import torch
+    Phi, y, lam = problem(M_nodes=250, k=35, seed=7)
+    th = (2.0, 3.0, 1.5, 0.05)
+    e = engine(Phi, y, lam, 777.0, "function", theta=th)
+    Ps = np.random.default_rng(1).normal(size=(40, 35))
+    m, v = e.predict(torch.as_tensor(Ps, device="cuda"))
+    S, _ = cpu.spectral_filter(lam, th[0], th[1], th[2], 777.0)
+    mr, vr = cpu.gpr_predict(Phi, y, S, th[3], Ps)
+    # FP64 both sides; condition of K_y ~1e6 bounds the agreement
+    m = m.cpu().numpy()
+    assert np.linalg.norm(m - mr.ravel()) / np.linalg.norm(mr) < 1e-7
+    np.testing.assert_allclose(v.cpu().numpy(), vr.ravel(), rtol=1e-6, atol=1e-10)
+
+
+def test_mixed_precision_function_space_within_1e4():
+    """FP32 Cholesky + FP64 iterative refinement vs the FP64 form."""
+    import torch
+    Phi, y, lam = problem(M_nodes=500, k=50, seed=11)
+    a = engine(Phi, y, lam, 3000.0, "function")
+    b = engine(Phi, y, lam, 3000.0, "function", precision="mixed")
+    oa, ob = a.evaluate().cpu().numpy(), b.evaluate().cpu().numpy()
+    assert abs(oa[0] - ob[0]) < 1e-8 * abs(oa[0])
+    np.testing.assert_allclose(ob[1:5], oa[1:5], rtol=1e-4, atol=1e-6 * np.abs(oa[1:5]).max())
+    Ps = torch.as_tensor(np.random.default_rng(2).normal(size=(30, 50)), device="cuda")
+    ma, _ = a.predict(Ps)
+    mb, _ = b.predict(Ps)
+    assert (torch.linalg.norm(ma - mb) / torch.linalg.norm(ma)).item() < 1e-4
+
+
+def test_adam_step_is_device_resident_and_decreases_nlml():
+    Phi, y, lam = problem(M_nodes=200, k=30, seed=13)
+    e = engine(Phi, y, lam, 900.0, "weight")
+    f0 = e.nlml_grad_u(e.u.cpu().numpy())[0]
+    e.adam_setup()
+    for _ in range(50):
+        e.adam_step(lr=0.05)
+    f1 = e.nlml_grad_u(e.u.cpu().numpy())[0]
+    assert f1 < f0
+
+
+def test_lbfgs_reaches_oracle_optimum():
+    Phi, y, lam = problem(M_nodes=150, k=25, seed=17)
+    e = engine(Phi, y, lam, 600.0, "function", theta=(1.5, 5.0, 1.0, 1.0))
+    res = e.fit_lbfgs(maxiter=200)
+    th = e.theta_host()
+    th_ref, res_ref = cpu.fit_lbfgs(Phi, y, lam, 600.0, maxiter=200)
+    assert abs(res.fun - res_ref.fun) < 1e-6 * abs(res_ref.fun)
+    np.testing.assert_allclose(th, th_ref, rtol=1e-3)

@@ -1,0 +1,257 @@
+"""CPU tests: pin the oracle against the reference's golden vectors, library
+routines, closed forms, finite differences and brute force (no GPU)."""
+import numpy as np
+import pytest
+import scipy.linalg
+import scipy.sparse as sp
+
+from oracle import cpu
+from tests.util import golden_cases, sphere, torus
+
+
+# ---------------------------------------------------------------- a1 FPS
+@pytest.mark.parametrize("n,D,spacing,N,start", [(400, 3, 0.1, None, 0), (300, 8, 0.2, None, 5),
+                                                 (500, 3, 0.1, 120, 3)])
+def test_fps_direct_matches_sklearn_formula(n, D, spacing, N, start):
+    x = np.random.default_rng(n).normal(size=(n, D))
+    p1, l1 = cpu.fps(x, N, spacing, start)
+    p2, l2 = cpu.fps(x, N, spacing, start, metric="sklearn")
+    np.testing.assert_array_equal(p1, p2)
+    np.testing.assert_allclose(l1, l2, rtol=1e-12, atol=1e-12)
+
+
+def test_fps_brute_force_tiny():
+    x = np.random.default_rng(0).normal(size=(12, 2))
+    Dm = np.sqrt(((x[:, None] - x[None]) ** 2).sum(-1))
+    p, l = cpu.fps(x, N=12)
+    sel = [0]
+    for i in range(1, 12):  # farthest from the selected set, lowest index on ties
+        dmin = Dm[:, sel].min(1)
+        sel.append(int(np.argmax(dmin)))
+    np.testing.assert_array_equal(p, sel)
+    assert np.all(np.diff(l[1:]) <= 1e-15)
+    assert abs(cpu.diameter(x) - Dm.max()) < 1e-12
+
+
+def test_fps_spacing_stop_rule():
+    x = torus(3000, 1)
+    p, l = cpu.fps(x, spacing=0.05)
+    diam = cpu.diameter(x)
+    assert np.all(l[1:] / diam >= 0.05)
+    p_all, l_all = cpu.fps(x, N=len(p) + 1)
+    assert l_all[len(p)] / diam < 0.05  # the dropped point triggered the stop
+
+
+# ---------------------------------------------------------------- a2 graph
+@pytest.mark.parametrize("n,D,k", [(500, 3, 10), (400, 32, 20), (300, 8, 15)])
+def test_knn_graph_matches_sklearn_networkx(n, D, k):
+    x = np.random.default_rng(n + k).normal(size=(n, D))
+    rp, c = cpu.sym_csr(cpu.knn(x, k))
+    rp2, c2, data = cpu.graph_csr_networkx(x, k)
+    np.testing.assert_array_equal(rp, rp2)
+    np.testing.assert_array_equal(c, c2)
+    assert np.all(data == 1.0)
+
+
+def test_knn_brute_force_ties():
+    g = np.stack(np.meshgrid(np.arange(5), np.arange(5), indexing="ij"), -1).reshape(-1, 2) * 1.0
+    nbr = cpu.knn(g, 4)
+    for i in range(len(g)):
+        d2 = ((g - g[i]) ** 2).sum(1)
+        d2[i] = np.inf
+        order = np.lexsort((np.arange(len(g)), d2))[:4]
+        np.testing.assert_array_equal(nbr[i], order)
+
+
+# ---------------------------------------------------------------- a3 heap + gauges
+@pytest.mark.parametrize("case", golden_cases())
+def test_heap_gauges_match_reference_golden(case):
+    """Golden vectors written by the reference's own compiled ptu_dijkstra."""
+    z = np.load(case)
+    X, k, K = z["X"], int(z["k"]), int(z["K"])
+    rp, c = cpu.sym_csr(cpu.knn(X, k))
+    knn = sp.csr_matrix((np.ones(len(z["knn_indices"])), z["knn_indices"], z["knn_indptr"]))
+    S = knn.maximum(knn.T).tocsr()
+    S.sort_indices()
+    np.testing.assert_array_equal(rp, S.indptr)
+    np.testing.assert_array_equal(c, S.indices)
+    T, Sig = cpu.tangent_frames(X, rp, c, X.shape[1], K)
+    np.testing.assert_allclose(Sig, z["Sigma"], rtol=1e-10, atol=1e-13)
+    d = int(z["dim"])
+    assert cpu.manifold_dimension(Sig, 0.8) == d
+    P = np.einsum("nij,nkj->nik", T[:, :, :d], T[:, :, :d])
+    Pr = np.einsum("nij,nkj->nik", z["tangents"][:, :, :d], z["tangents"][:, :, :d])
+    assert np.abs(P - Pr).max() < 1e-10
+    # connections from the reference's gauges vs the reference's R
+    R = cpu.connections(np.ascontiguousarray(z["tangents"][:, :, :d]), rp, c)
+    coo = cpu.connections_coo(R, rp, c)
+    order = np.lexsort((coo.col, coo.row))
+    oref = np.lexsort((z["R_col"], z["R_row"]))
+    np.testing.assert_array_equal(coo.row[order], z["R_row"][oref])
+    np.testing.assert_array_equal(coo.col[order], z["R_col"][oref])
+    np.testing.assert_allclose(coo.data[order], z["R_data"][oref], atol=1e-12)
+
+
+def test_heap_neighbourhood_is_a_dijkstra_prefix():
+    """Whatever the tie order, the K+1 popped nodes are a BFS-ball prefix."""
+    X = sphere(300, 2)
+    rp, c = cpu.sym_csr(cpu.knn(X, 6))
+    nbh, _ = cpu.geo_nbh(X, rp, c, 12, centred=False)
+    A = sp.csr_matrix((np.ones(len(c)), c, rp))
+    from scipy.sparse.csgraph import shortest_path
+    H = shortest_path(A, unweighted=True)
+    for i in range(300):
+        h = H[i, nbh[i]]
+        assert nbh[i][0] == i and np.all(np.diff(h) >= 0)
+        assert np.sum(H[i] < h.max()) <= 13  # every strictly closer node is included
+        assert len(set(nbh[i].tolist())) == 13
+
+
+def test_tangent_frames_validation():
+    X = sphere(50, 0)
+    rp, c = cpu.sym_csr(cpu.knn(X, 5))
+    with pytest.raises(ValueError):
+        cpu.tangent_frames(X, rp, c, 3, 50)
+    with pytest.raises(ValueError):
+        cpu.tangent_frames(X, rp, c, 3, 2)
+    with pytest.raises(ValueError):
+        cpu.tangent_frames(X, rp, c, 4, 10)
+    P = np.c_[np.random.default_rng(0).normal(size=(80, 2)), np.zeros(80)]
+    rp, c = cpu.sym_csr(cpu.knn(P, 8))
+    with pytest.raises(RuntimeError):
+        cpu.tangent_frames(P, rp, c, 3, 12)
+
+
+# ---------------------------------------------------------------- a5-a7
+def test_connections_polar_and_invariants():
+    X = sphere(300, 5)
+    rp, c = cpu.sym_csr(cpu.knn(X, 8))
+    T, _ = cpu.tangent_frames(X, rp, c, 3, 12)
+    T2 = np.ascontiguousarray(T[:, :, :2])
+    R = cpu.connections(T2, rp, c)
+    rows = np.repeat(np.arange(300), np.diff(rp))
+    M = np.einsum("eki,ekj->eij", T2[rows], T2[c])
+    for e in range(0, len(c), 97):
+        U, _ = scipy.linalg.polar(M[e])
+        np.testing.assert_allclose(R[e], U, atol=1e-12)
+    np.testing.assert_allclose(np.einsum("eji,ejk->eik", R, R), np.broadcast_to(np.eye(2), R.shape),
+                               atol=1e-12)
+    diag = rows == c
+    np.testing.assert_allclose(R[diag], np.broadcast_to(np.eye(2), R[diag].shape), atol=1e-12)
+    # R_ji = R_ij^T
+    key = {(i, j): e for e, (i, j) in enumerate(zip(rows, c))}
+    for e in range(0, len(c), 53):
+        i, j = rows[e], c[e]
+        np.testing.assert_allclose(R[key[(j, i)]], R[e].T, atol=1e-12)
+
+
+def test_laplacians_vs_networkx_and_invariants():
+    import networkx as nx
+    X = sphere(400, 6)
+    rp, c = cpu.sym_csr(cpu.knn(X, 9))
+    L = cpu.laplacian(rp, c)
+    A = sp.csr_matrix((np.ones(len(c)), c, rp))
+    G = nx.from_scipy_sparse_array(A)
+    Lnx = sp.csr_matrix(nx.laplacian_matrix(G), dtype=np.float64)
+    assert abs(L - Lnx).max() == 0
+    assert np.abs(L @ np.ones(400)).max() == 0
+    T, _ = cpu.tangent_frames(X, rp, c, 3, 13)
+    R = cpu.connections(np.ascontiguousarray(T[:, :, :2]), rp, c)
+    Lc = cpu.connection_laplacian(rp, c, R)
+    assert abs(Lc - Lc.T).max() < 1e-12
+    assert np.linalg.eigvalsh(Lc.toarray()).min() > -1e-10
+    Bl = cpu.lc_blocks(rp, c, R)
+    Lb = sp.bsr_matrix((Bl, c, rp), shape=Lc.shape).tocsr()
+    assert abs(Lb - Lc).max() < 1e-14
+
+
+# ---------------------------------------------------------------- a8 spectrum
+def test_spectrum_ring_closed_form():
+    n = 60
+    rows = np.arange(n)
+    A = sp.csr_matrix((np.ones(2 * n), (np.r_[rows, rows], np.r_[(rows + 1) % n, (rows - 1) % n])),
+                      shape=(n, n))
+    L = sp.diags(np.asarray(A.sum(1)).ravel()) - A
+    w, V = cpu.compute_spectrum(sp.csr_matrix(L), 9)
+    ref = np.sort(2 - 2 * np.cos(2 * np.pi * np.arange(n) / n))[:9]
+    np.testing.assert_allclose(w, ref, atol=1e-10)
+    np.testing.assert_allclose(np.linalg.norm(V, axis=0), np.sqrt(n))  # geometry.py:75
+    with pytest.raises(TypeError):
+        cpu.compute_spectrum(sp.csr_matrix(L), n - 1)
+
+
+def test_spectrum_eigsh_vs_eigh():
+    X = sphere(300, 7)
+    rp, c = cpu.sym_csr(cpu.knn(X, 8))
+    L = cpu.laplacian(rp, c)
+    w1, _ = cpu.compute_spectrum(L, 20)
+    w2, _ = cpu.compute_spectrum(L, 20, dense=True)
+    np.testing.assert_allclose(w1, w2, atol=1e-9)
+
+
+# ---------------------------------------------------------------- a11 smoothing
+def test_heat_expm_equals_eigen_form_and_preserves_constants():
+    X = sphere(150, 8)
+    rp, c = cpu.sym_csr(cpu.knn(X, 8))
+    L = cpu.laplacian(rp, c)
+    x = np.random.default_rng(0).normal(size=(150, 2))
+    np.testing.assert_allclose(cpu.heat(L, 3.0, x, "expm"), cpu.heat(L, 3.0, x, "eigh"), atol=1e-10)
+    np.testing.assert_allclose(cpu.heat(L, 50.0, np.ones((150, 1)), "expm"), 1.0, atol=1e-10)
+
+
+# ---------------------------------------------------------------- a12-a16 GP
+def test_spectral_filter_finite_differences():
+    lam = np.linspace(0, 6, 40)
+    for typ in ("matern", "se"):
+        th = np.array([1.7, 2.3, 0.9])
+        S, dS = cpu.spectral_filter(lam, *th, 500.0, typ)
+        assert abs(S.sum() - 500.0 * th[2]) < 1e-9
+        for p in range(3):
+            h = 1e-6
+            tp, tm = th.copy(), th.copy()
+            tp[p] += h
+            tm[p] -= h
+            fd = (cpu.spectral_filter(lam, *tp, 500.0, typ)[0] - cpu.spectral_filter(lam, *tm, 500.0, typ)[0]) / (2 * h)
+            np.testing.assert_allclose(dS[p], fd, rtol=1e-6, atol=1e-8)
+
+
+def test_gpr_nlml_vs_multivariate_normal_and_fd():
+    from scipy.stats import multivariate_normal
+    rng = np.random.default_rng(3)
+    Phi = rng.normal(size=(60, 12))
+    y = rng.normal(size=(60, 1))
+    S = rng.uniform(0.1, 2.0, size=12)
+    nlml = cpu.gpr_nlml(Phi, y, S, 0.3)
+    ref = -multivariate_normal(mean=np.zeros(60), cov=(Phi * S) @ Phi.T + 0.3 * np.eye(60)).logpdf(y.ravel())
+    assert abs(nlml - ref) < 1e-9 * abs(ref)
+    gS, gn = cpu.gpr_grad(Phi, y, S, 0.3)
+    h = 1e-6
+    for j in (0, 5, 11):
+        Sp, Sm = S.copy(), S.copy()
+        Sp[j] += h
+        Sm[j] -= h
+        fd = (cpu.gpr_nlml(Phi, y, Sp, 0.3) - cpu.gpr_nlml(Phi, y, Sm, 0.3)) / (2 * h)
+        assert abs(fd - gS[j]) < 1e-6
+    fd = (cpu.gpr_nlml(Phi, y, S, 0.3 + h) - cpu.gpr_nlml(Phi, y, S, 0.3 - h)) / (2 * h)
+    assert abs(fd - gn) < 1e-6
+
+
+def test_gpr_predict_conditioning_identity():
+    """Predicting at the training inputs with noise -> 0 interpolates y."""
+    rng = np.random.default_rng(4)
+    Phi = rng.normal(size=(20, 40))
+    y = rng.normal(size=(20, 1))
+    S = np.ones(40)
+    m, v = cpu.gpr_predict(Phi, y, S, 1e-10, Phi)
+    np.testing.assert_allclose(m.ravel(), y.ravel(), atol=1e-6)
+    assert np.abs(v).max() < 1e-6
+
+
+def test_train_test_split_rule_matches_sklearn():
+    from sklearn.model_selection import train_test_split
+    for n, seed in ((10, 0), (1000, 0), (37, 5)):
+        idx = np.arange(n)
+        tr, te = train_test_split(idx, test_size=0.2, random_state=seed)
+        tr2, te2 = cpu.train_test_split_indices(n, 0.2, seed)
+        np.testing.assert_array_equal(tr, tr2)
+        np.testing.assert_array_equal(te, te2)

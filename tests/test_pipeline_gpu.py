@@ -1,0 +1,85 @@
+"""End-to-end drop-in pipeline on the GPU vs the FP64 oracle pipeline:
+create_data_object -> random_vector_field -> smooth_vector_field -> fit ->
+transform, on a C1-style sphere at reduced n so the dense-expm oracle runs
+in seconds."""
+import numpy as np
+import pytest
+
+from oracle import cpu
+from tests.util import sphere
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def pair():
+    import rvgp_amd as RVGP
+    X = sphere(800, 0)
+    d = RVGP.create_data_object(X, n_neighbors=10, n_eigenpairs=30, verbose=False)
+    o = cpu.Data(X, n_neighbors=10, n_eigenpairs=30)
+    return d, o, X
+
+
+def test_data_object_matches_oracle(pair):
+    d, o, X = pair
+    assert d.n == 800 and d.dim_man == o.dim_man == 2
+    np.testing.assert_allclose(d.evals_Lc, o.evals_Lc, rtol=1e-6, atol=1e-9)
+    np.testing.assert_allclose(d.evals_L[1:], o.evals_L[1:], rtol=1e-6)
+    # gauge-invariant: projectors onto tangent planes
+    P = np.einsum("nij,nkj->nik", d.gauges, d.gauges)
+    Po = np.einsum("nij,nkj->nik", o.gauges, o.gauges)
+    assert np.abs(P - Po).max() < 1e-10
+    assert d.evecs_Lc.shape == (800 * 3, 30)
+    # parallel transport in ambient coordinates, T_i R_ij T_j^T, is gauge invariant
+    rows = np.repeat(np.arange(800), np.diff(o.rowptr))
+    Rg = d._R.cpu().numpy()
+    amb = np.einsum("eij,ejk,elk->eil", d.gauges[rows], Rg, d.gauges[o.cols])
+    amb_o = np.einsum("eij,ejk,elk->eil", o.gauges[rows], o.Rblocks, o.gauges[o.cols])
+    assert np.abs(amb - amb_o).max() < 1e-9
+    assert d.R.shape == (1600, 1600)
+    assert d.G.number_of_nodes() == 800
+
+
+def test_random_and_smoothed_field_match_oracle(pair):
+    d, o, X = pair
+    # projections and the smoothed ambient field are gauge invariant
+    d.random_vector_field(seed=1)
+    o.random_vector_field(seed=1)
+    np.testing.assert_allclose(d.vectors, o.vectors, rtol=1e-12, atol=1e-13)
+    d.smooth_vector_field(t=10)
+    o.smooth_vector_field(t=10, method="expm")
+    np.testing.assert_allclose(d.vectors, o.vectors, rtol=0, atol=1e-9 * np.abs(o.vectors).max())
+
+
+def test_fit_and_transform_match_oracle(pair):
+    import rvgp_amd as RVGP
+    d, o, X = pair
+    rng = np.random.default_rng(4)
+    train_ind = rng.choice(np.arange(800), size=400, replace=False)
+    gp = RVGP.fit(d, train_ind=train_ind, epochs=30, verbose=False)
+    test_ind = [int(i) for i in range(800) if i not in set(train_ind.tolist())]
+    mean, var = gp.transform(d, test_ind)
+    assert mean.shape == (len(test_ind), 3) and var.shape == (len(test_ind), 3)
+    # the oracle at the GPU's fitted hyperparameters, on the oracle's own
+    # eigenpairs and vectors (gauge-invariant ambient predictions)
+    th = gp.engine.theta_host()
+    tr, _ = cpu.train_test_split_indices(400, 0.2, 0)
+    nodes = train_ind[tr]
+    Phi = o.evecs_Lc.reshape(800, -1)[nodes].reshape(-1, 30)
+    y = o.vectors[nodes].reshape(-1, 1)
+    S, _ = cpu.spectral_filter(o.evals_Lc, th[0], th[1], th[2], 800 * 3)
+    Ps = o.evecs_Lc.reshape(800, -1)[test_ind].reshape(-1, 30)
+    mr, vr = cpu.gpr_predict(Phi, y, S, th[3], Ps)
+    rel = np.linalg.norm(mean.ravel() - mr.ravel()) / np.linalg.norm(mr)
+    assert rel < 1e-4, rel
+    with pytest.raises(UnboundLocalError):
+        gp.transform(d, np.asarray(test_ind))  # np.int64 dispatch (main.py:104-109)
+
+
+def test_fit_reference_quirks(pair):
+    import rvgp_amd as RVGP
+    d, _, _ = pair
+    with pytest.raises(AttributeError):
+        RVGP.fit(d, train_ind=np.arange(100), kernel_variance=1.0, epochs=1, verbose=False)
+    with pytest.raises(NotImplementedError):
+        RVGP.fit(d, train_ind=np.arange(100), n_inducing_points=10, epochs=1, verbose=False)

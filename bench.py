@@ -1,0 +1,287 @@
+#!/usr/bin/env python3
+"""Headline benchmark: full RVGP fit on the 3-torus in R^32 (n = 100,000,
+k = 20, dim_man = 3, 500 eigenpairs, 30% of nodes train), mixed precision +
+adjoint vs the plain FP64 GPU form, on one MI355X.
+
+A "step" is one full fit: kNN graph, heap neighbourhoods + gauges,
+connections, block-CSR Laplacians, both eigensolves (create_data_object),
+then RVGP.fit with 200 Adam steps on the NLML through the hand-written
+adjoint.  Points and the target tangent field are synthetic (seeded) and
+resident in HBM before the timed region.
+
+Prints ONE JSON line:
+  value          wall time of the mixed-precision fit (s, lower is better)
+  fp64_s         wall time of the FP64 GPU fit on the same input
+  rel_err_pred   |f_mixed - f_fp64| / |f_fp64| over test-node predictions
+  roofline       dominant kernel (the FP32 block-SpMM of the Lc eigensolve):
+                 algorithmic bytes / HIP-event-timed duration vs 8 TB/s
+  cpu_baseline   the FP64 CPU oracle (oracle/) on a bounded sample of the
+                 same workload, with the GPU time on that sample beside it
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "fit wall-time @n=1e5,500 eigs: mixed-prec+adjoint vs FP64 GPU at <=1e-4 rel err"
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+# ---------------------------------------------------------------- workload
+def flat_torus3(n, D=32, seed=0, noise=0.01):
+    """Angles a ~ U[0, 2pi)^3, E = [cos a, sin a] in R^6, mapped into R^D by the
+    first 6 rows of a seeded orthogonal Q, plus isotropic N(0, noise^2)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    a = rng.uniform(0, 2 * np.pi, (n, 3))
+    E = np.concatenate([np.cos(a), np.sin(a)], 1)
+    Qm, Rq = np.linalg.qr(np.random.default_rng(seed + 1000).normal(size=(D, D)))
+    Qm = Qm * np.sign(np.diag(Rq))
+    X = E @ Qm[:6] + noise * rng.normal(size=(n, D))
+    return X, a, Qm[:6]
+
+
+def torus_field(a, Q6):
+    """A smooth tangent field sum_i f_i(a) dE/da_i pushed into R^D."""
+    import numpy as np
+    f = np.stack([np.sin(a[:, 1]) + 0.5 * np.cos(a[:, 2]), np.cos(a[:, 0] + a[:, 2]),
+                  np.sin(a[:, 0]) * np.cos(a[:, 1])], 1)
+    v6 = np.zeros((len(a), 6))
+    for i in range(3):
+        v6[:, i] += -np.sin(a[:, i]) * f[:, i]
+        v6[:, 3 + i] += np.cos(a[:, i]) * f[:, i]
+    return v6 @ Q6
+
+
+def run_fit(X, V, train, precision, args, timings):
+    import torch
+    import rvgp_amd as RVGP
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    d = RVGP.create_data_object(X, vectors=V, n_neighbors=args.k_nn, n_eigenpairs=args.k_eig,
+                                precision=precision, reference_semantics=False, verbose=False,
+                                timings=timings, eig_verbose=args.verbose)
+    t1 = time.perf_counter()
+    gp = RVGP.fit(d, train_ind=train, epochs=args.adam_steps, optimizer="adam", lr=0.05,
+                  precision=precision, verbose=False)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    timings["data_object"] = t1 - t0
+    timings["gp_fit"] = t2 - t1
+    return d, gp, t2 - t0
+
+
+def kernel_stats(rec):
+    out = {}
+    for (kind, key), s in rec.summary().items():
+        name = f"{kind}[{key[0]},b{key[1]}]" if isinstance(key, tuple) else f"{kind}[{key}]"
+        out[name] = {"count": s["count"], "avg_us": 1e3 * s["ms"] / max(1, s["count"]),
+                     "total_ms": s["ms"], "GBps": s["bytes"] / (s["ms"] * 1e6) if s["ms"] else 0.0,
+                     "TFLOPs": s["flops"] / (s["ms"] * 1e9) if s["ms"] else 0.0}
+    return out
+
+
+def traffic_from_profile():
+    """HBM bytes per launch of the dominant kernel from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, scaled by a plain copy
+    kernel), or None when no such summary is present."""
+    path = os.path.join(ROOT, "profiles", "spmm_wide_f32_b3_c3_pmc.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":")[1])
+    return None
+
+
+# ---------------------------------------------------------------- CPU baseline
+def cpu_baseline(args):
+    """FP64 oracle (oracle/cpu.py: C kNN + heap, LAPACK SVDs, eigsh, textbook
+    GPR) on the same generator at n = args.cpu_n: data object timed in full,
+    NLML + gradient evaluations timed and scaled to the fit's Adam steps."""
+    import numpy as np
+    from oracle import cpu
+    try:
+        from threadpoolctl import threadpool_info
+        cores = max([t.get("num_threads", 1) for t in threadpool_info()] + [1])
+    except Exception:
+        cores = os.cpu_count()
+    n = args.cpu_n
+    X, a, Q6 = flat_torus3(n, seed=0)
+    V = torus_field(a, Q6)
+    rng = np.random.default_rng(3)
+    train = rng.permutation(n)[: int(0.3 * n)]
+    k = args.cpu_k_eig
+    t0 = time.perf_counter()
+    o = cpu.Data(X, n_neighbors=args.k_nn, n_eigenpairs=k, allow_rank_deficient_k=True)
+    t_data = time.perf_counter() - t0
+    tr, _ = cpu.train_test_split_indices(len(train), 0.2, 0)
+    nodes = train[tr]
+    Phi = o.evecs_Lc.reshape(n, -1)[nodes].reshape(-1, k)
+    y = V[nodes].reshape(-1, 1)
+    evals = 2
+    t0 = time.perf_counter()
+    for _ in range(evals):
+        S, dS = cpu.spectral_filter(o.evals_Lc, 1.5, 5.0, 1.0, n * X.shape[1])
+        cpu.gpr_nlml(Phi, y, S, 1.0)
+        cpu.gpr_grad(Phi, y, S, 1.0)
+    t_eval = (time.perf_counter() - t0) / evals
+    total = t_data + args.adam_steps * t_eval
+    return {"value": total, "unit": "s per fit", "cores": int(cores), "kind": "port",
+            "sample": (f"oracle on the same generator at n={n} (k={args.k_nn}, {k} eigenpairs, "
+                       f"M={Phi.shape[0]} GP rows): data object {t_data:.2f}s timed in full; "
+                       f"NLML+gradient {t_eval:.2f}s per evaluation timed over {evals} and "
+                       f"scaled to {args.adam_steps} steps")}
+
+
+def gpu_same_sample(args):
+    import numpy as np
+    n = args.cpu_n
+    X, a, Q6 = flat_torus3(n, seed=0)
+    V = torus_field(a, Q6)
+    train = np.random.default_rng(3).permutation(n)[: int(0.3 * n)]
+    a2 = argparse.Namespace(**vars(args))
+    a2.k_eig = args.cpu_k_eig
+    run_fit(X, V, train, "mixed", a2, {})  # warm
+    _, _, t = run_fit(X, V, train, "mixed", a2, {})
+    return t
+
+
+# ---------------------------------------------------------------- main
+def main():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=1)
+    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--n", type=int, default=100000)
+    p.add_argument("--k-eig", dest="k_eig", type=int, default=500)
+    p.add_argument("--k-nn", dest="k_nn", type=int, default=20)
+    p.add_argument("--adam-steps", dest="adam_steps", type=int, default=200)
+    p.add_argument("--skip-fp64", action="store_true")
+    p.add_argument("--skip-cpu", action="store_true")
+    p.add_argument("--cpu-n", dest="cpu_n", type=int, default=800)
+    p.add_argument("--cpu-k-eig", dest="cpu_k_eig", type=int, default=50)
+    p.add_argument("--verbose", action="store_true")
+    args = p.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
+              "design is described in DESIGN.md but not built.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from rvgp_amd import _lib, instrument
+    _lib.lib()
+    dev = _lib.device()
+
+    X, a, Q6 = flat_torus3(args.n, seed=0)
+    V = torus_field(a, Q6)
+    train = np.random.default_rng(3).permutation(args.n)[: int(0.3 * args.n)]
+    Xd = torch.as_tensor(X, device=dev)
+    Vd = torch.as_tensor(V, device=dev)
+    log(f"[bench] n={args.n} D={X.shape[1]} k_eig={args.k_eig} train={len(train)}")
+
+    for w in range(args.warmup):
+        log(f"[bench] warmup {w + 1}/{args.warmup} (mixed)")
+        run_fit(Xd, Vd, train, "mixed", args, {})
+
+    times, stages = [], {}
+    rec = instrument.Recorder()
+    for s in range(args.steps):
+        log(f"[bench] timed step {s + 1}/{args.steps} (mixed)")
+        torch.cuda.synchronize()
+        with rec if s == args.steps - 1 else _Null():
+            d_mx, gp_mx, t = run_fit(Xd, Vd, train, "mixed", args, stages)
+        torch.cuda.synchronize()
+        times.append(t)
+        log(f"[bench] mixed fit {t:.3f}s stages={ {k: round(v, 3) for k, v in stages.items()} }")
+    t_mixed = sum(times) / len(times)
+    kstats = kernel_stats(rec)
+
+    rng = np.random.default_rng(5)
+    test_nodes = [int(i) for i in rng.choice(np.setdiff1d(np.arange(args.n), train),
+                                             size=min(10000, args.n - len(train)), replace=False)]
+    f_mx, _ = gp_mx.transform(d_mx, test_nodes)
+    result_extra = {}
+    t_fp64 = None
+    if not args.skip_fp64:
+        log("[bench] FP64 fit")
+        st64 = {}
+        d64, gp64, t_fp64 = run_fit(Xd, Vd, train, "fp64", args, st64)
+        log(f"[bench] fp64 fit {t_fp64:.3f}s stages={ {k: round(v, 3) for k, v in st64.items()} }")
+        f64, _ = gp64.transform(d64, test_nodes)
+        w64 = d64._evals_Lc.cpu().numpy()
+        wmx = d_mx._evals_Lc.cpu().numpy()
+        ub = d64._Lcb.gershgorin_bound()
+        result_extra = {
+            "fp64_s": t_fp64,
+            "speedup_vs_fp64": t_fp64 / t_mixed,
+            "rel_err_pred": float(np.linalg.norm(f_mx - f64) / np.linalg.norm(f64)),
+            "eig_rel_err_max": float(np.max(np.abs(wmx - w64) / np.maximum(np.abs(w64), 1e-9 * ub))),
+            "nlml_rel_err": float(abs(gp_mx.training_loss() - gp64.training_loss())
+                                  / abs(gp64.training_loss())),
+            "stages_fp64_s": {k: round(v, 4) for k, v in st64.items()},
+        }
+        del d64, gp64
+
+    # roofline of the dominant kernel: FP32 block SpMM (b = dim_man) of the
+    # Lc eigensolve, measured over every launch of that kernel in the timed fit
+    dm = d_mx.dim_man
+    key = f"spmm_wide[float32,b{dm}]"
+    ks = kstats.get(key, {"count": 0, "avg_us": 0.0, "GBps": 0.0})
+    roof = {"kernel": key, "bound": "hbm", "achieved": round(ks["GBps"], 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(ks["GBps"] / HBM_PEAK_GBS, 4),
+            "traffic": traffic_from_profile(), "launches": ks["count"],
+            "avg_us": round(ks["avg_us"], 2)}
+    gram = kstats.get("gemm[f32acc64]", None)
+
+    cpu_b = None
+    if not args.skip_cpu:
+        log("[bench] CPU baseline (oracle)")
+        cpu_b = cpu_baseline(args)
+        cpu_b["gpu_same_sample_s"] = gpu_same_sample(args)
+
+    out = {
+        "metric": METRIC, "value": round(t_mixed, 4), "unit": "s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * t_mixed, 1),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
+        "data": "synthetic (seeded flat 3-torus in R^32 + analytic tangent field)",
+        "config": {"workload": (f"3-torus in R^32, n={args.n}, k={args.k_nn}, dim_man={dm}, "
+                                f"{args.k_eig} eigenpairs, 30% train (80/20 re-split), "
+                                f"{args.adam_steps} adjoint Adam steps"),
+                   "n": args.n, "k_nn": args.k_nn, "n_eigenpairs": args.k_eig,
+                   "adam_steps": args.adam_steps, "parallelism": "single GPU"},
+        **result_extra,
+        "stages_mixed_s": {k: round(v, 4) for k, v in stages.items()},
+        "eig_stats_mixed": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv)
+                                for kk, vv in v.items()} for k, v in d_mx.eig_stats.items()},
+        "roofline": roof,
+        "gram_fp64_mfma": None if gram is None else {
+            "achieved": round(gram["TFLOPs"], 2), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(gram["TFLOPs"] / FP64_PEAK_TFLOPS, 4), "launches": gram["count"]},
+        "kernels": {k: {kk: round(vv, 3) for kk, vv in v.items()} for k, v in kstats.items()},
+        "cpu_baseline": cpu_b,
+    }
+    print(json.dumps(out))
+
+
+class _Null:
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        return False
+
+
+if __name__ == "__main__":
+    main()

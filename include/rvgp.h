@@ -190,6 +190,11 @@ int rvgp_gemm(int mode, int transA, int transB, int M, int N, int K, double alph
               int lda, const void* B, int ldb, const double* kscale, double beta, void* C, int ldc,
               int split_k, void* ws, size_t ws_bytes, void* stream);
 
+/* Workspace bytes rvgp_gemm needs for this shape; split_k <= 0 lets the
+ * library choose the split (from the shape only, so results are
+ * reproducible).  0 when no split is used. */
+size_t rvgp_gemm_ws_bytes(int mode, int M, int N, int K, int split_k);
+
 /* In-place lower Cholesky A = L L^T of an n x n row-major matrix (f64 or
  * f32 by `f64` flag); the strictly upper triangle is left unspecified.  A
  * non-positive pivot sets status[0] = RVGP_ST_NOT_SPD (the condition that

@@ -92,7 +92,13 @@ __global__ __launch_bounds__(256) void spmm_wide(
     int ldacc, T delta) {
     constexpr int WPB = 4;
     const int lane = threadIdx.x & 63;
-    const int row = __builtin_amdgcn_readfirstlane(blockIdx.x * WPB + (threadIdx.x >> 6));
+    // XCD-aware mapping: workgroups are dealt round-robin to the 8 XCDs, so
+    // workgroup w runs on XCD w % 8; give each XCD one contiguous range of
+    // block rows (gridDim.x is a multiple of 8) so that its L2 serves the
+    // neighbouring X rows of consecutive rows.
+    const int per = gridDim.x >> 3;
+    const int lw = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    const int row = __builtin_amdgcn_readfirstlane(lw * WPB + (threadIdx.x >> 6));
     if (row >= nb) return;
     const int c0 = blockIdx.y * (64 * CPL) + lane;
     T y[B][CPL];
@@ -141,6 +147,100 @@ __global__ __launch_bounds__(256) void spmm_wide(
     }
 }
 
+// Broadcast lane `l` of v (l wave-uniform) -- v_readlane into an SGPR.
+__device__ __forceinline__ float bcast(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ double bcast(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Wide SpMM, latency-hiding form.  One wavefront per (block row, column
+// chunk).  The row's column indices and block values (up to 64 blocks at a
+// time) are fetched with ONE coalesced load per register (lane l holds entry
+// l of the flattened list) and broadcast with v_readlane; the block loop is
+// fully unrolled, so every lane/register index is a compile-time constant
+// and the X gathers of different blocks are independent loads the hardware
+// keeps in flight together, instead of a dependent index->address->load
+// chain per block.
+template <typename T, int B, int CPL>
+__global__ __launch_bounds__(256) void spmm_wide2(
+    int nb, const int* __restrict__ rowptr, const int* __restrict__ cols,
+    const T* __restrict__ vals, const T* __restrict__ X, int ldx, int r, T alpha, T beta,
+    const T* __restrict__ Z, int ldz, T gamma, T* __restrict__ Y, int ldy, T* __restrict__ acc,
+    int ldacc, T delta) {
+    constexpr int WPB = 4;
+    constexpr int BB = B * B;
+    const int lane = threadIdx.x & 63;
+    const int per = gridDim.x >> 3;
+    const int lw = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    const int row = __builtin_amdgcn_readfirstlane(lw * WPB + (threadIdx.x >> 6));
+    if (row >= nb) return;
+    const int c0 = blockIdx.y * (64 * CPL) + lane;
+    T y[B][CPL];
+#pragma unroll
+    for (int a = 0; a < B; ++a)
+#pragma unroll
+        for (int t = 0; t < CPL; ++t) y[a][t] = T(0);
+    const int e0 = rowptr[row], e1 = rowptr[row + 1];
+    for (int base = e0; base < e1; base += 64) {
+        const int cnt = min(64, e1 - base);
+        const int mycol = lane < cnt ? cols[base + lane] : 0;
+        T mv[BB];
+#pragma unroll
+        for (int q = 0; q < BB; ++q) {
+            const int f = lane + 64 * q;
+            mv[q] = f < cnt * BB ? vals[(size_t)base * BB + f] : T(0);
+        }
+#pragma unroll
+        for (int e = 0; e < 64; ++e) {
+            if (e < cnt) {
+                const int j = __builtin_amdgcn_readlane(mycol, e);
+                T v[B][B];
+#pragma unroll
+                for (int a = 0; a < B; ++a)
+#pragma unroll
+                    for (int bb = 0; bb < B; ++bb) {
+                        const int f = e * BB + a * B + bb;  // constant after unrolling
+                        v[a][bb] = bcast(mv[f >> 6], f & 63);
+                    }
+                const T* xr = X + (size_t)j * B * ldx;
+#pragma unroll
+                for (int t = 0; t < CPL; ++t) {
+                    const int c = c0 + 64 * t;
+                    if (c < r) {
+                        T xb[B];
+#pragma unroll
+                        for (int bb = 0; bb < B; ++bb) xb[bb] = xr[(size_t)bb * ldx + c];
+#pragma unroll
+                        for (int a = 0; a < B; ++a)
+#pragma unroll
+                            for (int bb = 0; bb < B; ++bb) y[a][t] += v[a][bb] * xb[bb];
+                    }
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < B; ++a) {
+        const size_t rr = (size_t)(row * B + a);
+#pragma unroll
+        for (int t = 0; t < CPL; ++t) {
+            const int c = c0 + 64 * t;
+            if (c < r) {
+                T out = alpha * y[a][t];
+                if (beta != T(0)) out += beta * X[rr * ldx + c];
+                if (Z != nullptr && gamma != T(0)) out += gamma * Z[rr * ldz + c];
+                Y[rr * ldy + c] = out;
+                if (acc != nullptr) acc[rr * ldacc + c] += delta * out;
+            }
+        }
+    }
+}
+
 template <typename T>
 static int spmm_dispatch(int nb, int b, const int* rowptr, const int* cols, const T* vals,
                          const T* X, int ldx, int r, T alpha, T beta, const T* Z, int ldz,
@@ -164,15 +264,26 @@ static int spmm_dispatch(int nb, int b, const int* rowptr, const int* cols, cons
         else RVGP_NARROW(3);
 #undef RVGP_NARROW
     } else {
-        constexpr int CPL = sizeof(T) == 4 ? 4 : 2;
-        dim3 g(cdiv(nb, 4), cdiv(r, 64 * CPL));
-#define RVGP_WIDE(BB)                                                                          \
-    hipLaunchKernelGGL((spmm_wide<T, BB, CPL>), g, dim3(256), 0, s, nb, rowptr, cols, vals, X, \
+        // columns per lane: balance the column chunks (at most 4 f32 / 2 f64)
+        const int cmax = sizeof(T) == 4 ? 4 : 2;
+        const int nchunks = (r + 64 * cmax - 1) / (64 * cmax);
+        const int cpl = (r + 64 * nchunks - 1) / (64 * nchunks);
+        dim3 g(8 * cdiv(cdiv(nb, 4), 8), nchunks);
+#define RVGP_WIDE2(BB, CC)                                                                      \
+    hipLaunchKernelGGL((spmm_wide2<T, BB, CC>), g, dim3(256), 0, s, nb, rowptr, cols, vals, X, \
                        ldx, r, alpha, beta, Z, ldz, gamma, Y, ldy, acc, ldacc, delta)
-        if (b == 1) RVGP_WIDE(1);
-        else if (b == 2) RVGP_WIDE(2);
-        else RVGP_WIDE(3);
-#undef RVGP_WIDE
+#define RVGP_WIDE2_B(CC)                 \
+    if (b == 1) RVGP_WIDE2(1, CC);       \
+    else if (b == 2) RVGP_WIDE2(2, CC);  \
+    else RVGP_WIDE2(3, CC);
+        if (cpl == 1) { RVGP_WIDE2_B(1) }
+        else if (cpl == 2) { RVGP_WIDE2_B(2) }
+        else if constexpr (sizeof(T) == 4) {
+            if (cpl == 3) { RVGP_WIDE2_B(3) }
+            else { RVGP_WIDE2_B(4) }
+        }
+#undef RVGP_WIDE2_B
+#undef RVGP_WIDE2
     }
     RVGP_LAUNCH_CHECK();
     return RVGP_OK;

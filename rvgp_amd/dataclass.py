@@ -17,6 +17,7 @@ import torch
 from . import _lib, geometry
 from .eigsolve import chfsi
 from .ptu_dijkstra import connections_device, tangent_frames_device
+from .reorder import morton_perm, permute_bsr, unpermute_rows
 from .smoothing import vector_diffusion_device
 from .sparse import BSR
 
@@ -25,7 +26,7 @@ class data:
     def __init__(self, vertices, vectors=None, dim_man=2, n_neighbors=10,
                  frac_geodesic_neighbours=1.5, explained_variance=0.8, n_eigenpairs=None, *,
                  precision="fp64", reference_semantics=True, eig_tol=1e-9, verbose=True,
-                 timings=None):
+                 timings=None, eig_verbose=False, reorder=True):
         """precision: "fp64" (all FP64) or "mixed" (FP32 eigensolver body with
         FP64 Rayleigh-Ritz and polish).  reference_semantics=False enables
         the documented extension for D > K (min(D, K+1) singular values)."""
@@ -78,11 +79,21 @@ class data:
         t0 = time.perf_counter()
         kL = N if n_eigenpairs is None else min(int(n_eigenpairs), N)
         kLc = N * dm if n_eigenpairs is None else min(int(n_eigenpairs), N * dm)
-        wL, VL, stL = chfsi(Lb, kL, precision=precision, tol=eig_tol)
+        if reorder:
+            # Morton renumbering for SpMM locality (same spectra; rows mapped back)
+            perm = morton_perm(Xd)
+            Lb_e, Lcb_e = permute_bsr(Lb, perm), permute_bsr(Lcb, perm)
+        else:
+            Lb_e, Lcb_e = Lb, Lcb
+        wL, VL, stL = chfsi(Lb_e, kL, precision=precision, tol=eig_tol, verbose=eig_verbose)
+        if reorder:
+            VL = unpermute_rows(VL, perm, 1)
         sync()
         t["eig_L"] = time.perf_counter() - t0
         t0 = time.perf_counter()
-        wLc, VLc, stLc = chfsi(Lcb, kLc, precision=precision, tol=eig_tol)
+        wLc, VLc, stLc = chfsi(Lcb_e, kLc, precision=precision, tol=eig_tol, verbose=eig_verbose)
+        if reorder:
+            VLc = unpermute_rows(VLc, perm, dm)
         sync()
         t["eig_Lc"] = time.perf_counter() - t0
         self.eig_stats = {"L": stL, "Lc": stLc}

@@ -1,7 +1,7 @@
 """Host wrappers of the matrix-core GEMM, Cholesky and triangular solves."""
 import torch
 
-from . import _lib
+from . import _lib, instrument
 
 GEMM_F64, GEMM_F32, GEMM_F32_ACC64, GEMM_BF16 = 0, 1, 2, 3
 
@@ -41,20 +41,24 @@ def gemm(A, B, C=None, transA=False, transB=False, alpha=1.0, beta=0.0, kscale=N
     assert C.dtype == want[1] and C.shape == (M, N)
     if kscale is not None:
         kscale = kscale.to(torch.float64).contiguous()
-    tiles = -(-M // 64) * -(-N // 64)
     if split_k is None:
-        split_k = 1
-        if tiles < 1024 and K >= 2048:
-            split_k = int(min(max(1, 2048 // tiles), K // 512, 256))
+        split_k = 0  # the library picks the K split from the shape
     ws = None
-    ws_bytes = 0
-    if split_k > 1:
-        ws_bytes = split_k * M * N * C.element_size()
+    ws_bytes = _lib.lib().rvgp_gemm_ws_bytes(mode, M, N, K, int(split_k))
+    if ws_bytes > 0:
         ws = torch.empty(ws_bytes, dtype=torch.uint8, device=A.device)
+    ev = instrument.begin()
     _lib.call("rvgp_gemm", mode, int(transA), int(transB), M, N, K, float(alpha), _lib.ptr(A),
               _ld(A), _lib.ptr(B), _ld(B), _lib.ptr(kscale), float(beta), _lib.ptr(C), _ld(C),
               int(split_k), _lib.ptr(ws), ws_bytes, _lib.stream())
+    if ev is not None:
+        instrument.end(ev, "gemm", MODE_NAMES[mode],
+                       (M * K + K * N) * A.element_size() + M * N * C.element_size(),
+                       2.0 * M * N * K)
     return C
+
+
+MODE_NAMES = {GEMM_F64: "f64", GEMM_F32: "f32", GEMM_F32_ACC64: "f32acc64", GEMM_BF16: "bf16"}
 
 
 def potrf(A, check=True):

@@ -27,6 +27,7 @@ Returns eigenvalues ascending and eigenvectors with unit 2-norm; the callers
 scale by sqrt(rows) as geometry.py:75 does.
 """
 import math
+import sys
 import time
 
 import numpy as np
@@ -111,11 +112,13 @@ def _deflate(Y, Xl):
     return Y
 
 
-def _run_phase(A, X, k, ub, wdt, ptol, max_iter, max_ratio, lock, fresh, stats, verbose, tag):
+def _run_phase(A, X, k, ub, wdt, ptol, max_iter, max_ratio, lock, fresh, stats, verbose, tag,
+               stall_exit=False, m_max=60):
     """Filter + Rayleigh-Ritz iterations on block X (work dtype wdt) until the
     k wanted pairs have residual/ub < ptol.  With ``lock``, converged leading
     pairs are frozen and deflated out of the filtered block."""
     p = X.shape[1]
+    history = []
     Xl = X[:, :0]
     thl = torch.zeros(0, dtype=torch.float64, device=X.device)
     Y = X
@@ -133,11 +136,17 @@ def _run_phase(A, X, k, ub, wdt, ptol, max_iter, max_ratio, lock, fresh, stats, 
         worst = float(res[:kk].max().item()) if kk > 0 else 0.0
         if verbose:
             print(f"[chfsi {tag}] it={it} locked={Xl.shape[1]} active={Xa.shape[1]} "
-                  f"worst_res={worst:.3e}")
+                  f"worst_res={worst:.3e} spmm={stats['spmm']}", file=sys.stderr, flush=True)
         if kk >= need and worst < ptol:
             X = torch.cat([Xl, Xa], 1)
             return X, torch.cat([thl, tha]), it
-        if it >= max_iter:
+        # a phase whose residuals have stopped falling (the FP32 rounding
+        # floor of the mixed form) hands over to the next phase
+        history.append(worst)
+        stalled = stall_exit and len(history) > 3 and min(history[-2:]) > 0.5 * min(history[:-2])
+        if it >= max_iter or stalled:
+            if stalled:
+                stats["stalled_" + tag] = worst
             X = torch.cat([Xl, Xa], 1)
             stats["unconverged"] = worst
             return X, torch.cat([thl, tha]), it
@@ -157,7 +166,7 @@ def _run_phase(A, X, k, ub, wdt, ptol, max_iter, max_ratio, lock, fresh, stats, 
         cut = float(tha[min(tha.numel(), Xa.shape[1]) - 1].item())
         low = float(tha[0].item())
         kk = min(k - Xl.shape[1], tha.numel())
-        m = _degree(tha, max(kk, 1), cut, ub, max_ratio)
+        m = _degree(tha, max(kk, 1), cut, ub, max_ratio, m_max=m_max)
         tf = time.perf_counter()
         Y = _cheb_filter(A, Xa, m, cut, ub, low)
         torch.cuda.synchronize()
@@ -191,21 +200,25 @@ def chfsi(A: BSR, k, precision="fp64", tol=1e-9, max_iter=300, guard=None, seed=
     t0 = time.perf_counter()
     X = X0.contiguous() if X0 is not None else fresh(p, torch.float64)
     if precision == "mixed":
-        # FP32 filter body with locking, to FP32-level residuals
-        X, theta, it = _run_phase(A, X.to(torch.float32).contiguous(), k, ub, torch.float32, 2e-6,
-                                  max_iter, 300.0, True, fresh, stats, verbose, "fp32")
+        # FP32 filter body with locking, down to the FP32 residual floor
+        X, theta, it = _run_phase(A, X.to(torch.float32).contiguous(), k, ub, torch.float32, 3e-6,
+                                  max_iter, 300.0, True, fresh, stats, verbose, "fp32",
+                                  stall_exit=True)
         stats["iters_fp32"] = it
         stats["t_fp32"] = time.perf_counter() - t0
-        # FP64 polish of the whole block (no locking): a few filter + RR passes
+        # FP64 polish with locking: the FP32 block is already accurate to
+        # ~1e-6, so one or two filter passes finish most pairs
         X, theta, it = _run_phase(A, X.double().contiguous(), k, ub, torch.float64, tol, 50, 1e4,
-                                  False, fresh, stats, verbose, "fp64-polish")
+                                  True, fresh, stats, verbose, "fp64-polish")
+        X, theta, _ = _run_phase(A, X, k, ub, torch.float64, tol, 3, 1e4, False, fresh, stats,
+                                 verbose, "fp64-final", m_max=16)
         stats["iters_polish"] = it
     elif precision == "fp64":
         X, theta, it = _run_phase(A, X.to(torch.float64).contiguous(), k, ub, torch.float64, tol,
                                   max_iter, 1e4, True, fresh, stats, verbose, "fp64")
         # one unlocked Rayleigh-Ritz over the whole block orders and refines
-        X, theta, _ = _run_phase(A, X, k, ub, torch.float64, tol, 20, 1e4, False, fresh, stats,
-                                 verbose, "fp64-final")
+        X, theta, _ = _run_phase(A, X, k, ub, torch.float64, tol, 3, 1e4, False, fresh, stats,
+                                 verbose, "fp64-final", m_max=16)
     else:
         raise ValueError("precision must be 'fp64' or 'mixed'")
     stats["t_total"] = time.perf_counter() - t0

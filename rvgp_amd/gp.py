@@ -66,11 +66,15 @@ class GPREngine:
             # function-space form it carries log det and diag(Phi^T K^-1 Phi)
             # in FP64 so that only the alpha solve uses the FP32 factor)
             if gram_mode is None:
-                gram_mode = dense.GEMM_F64 if precision == "fp64" else dense.GEMM_F32
+                # mixed: FP32 Phi tiles, products accumulated in FP64 on the
+                # matrix cores (an FP32-accumulated G over M = 768k rows costs
+                # ~1e-4 in the NLML)
+                gram_mode = dense.GEMM_F64 if precision == "fp64" else dense.GEMM_F32_ACC64
             self.gram_mode = gram_mode
             P = self.Phi if gram_mode == dense.GEMM_F64 else self.Phi.float()
+            acc64 = gram_mode in (dense.GEMM_F64, dense.GEMM_F32_ACC64)
             G = dense.gemm(P, P, transA=True, mode=gram_mode,
-                           out_dtype=torch.float64 if gram_mode == dense.GEMM_F64 else torch.float32)
+                           out_dtype=torch.float64 if acc64 else torch.float32)
             self.G = G.double().contiguous()
             self.G = 0.5 * (self.G + self.G.T)
             self.b = dense.gemm(self.Phi, self.y, transA=True)
@@ -206,8 +210,10 @@ class GPREngine:
         out = dense.gemm(self.Phi, t.contiguous())
         return out + self.s2 * x
 
-    def _refine(self, x, iters=3):
-        """FP64 iterative refinement of K_y x = y with the FP32 factor."""
+    def _refine(self, x, iters=8):
+        """FP64 iterative refinement of K_y x = y with the FP32 factor.  Each
+        sweep contracts the error by ~eps32 * cond(K_y); a fixed count keeps the
+        step free of host synchronisation (graph-capturable)."""
         for _ in range(iters):
             r = self.y - self._apply_Ky(x)
             self._solve_L(r)
@@ -285,3 +291,28 @@ class GPREngine:
         _lib.call("rvgp_adam_step", _lib.ptr(self.u), _lib.ptr(self.out[5:]), _lib.ptr(self.m1),
                   _lib.ptr(self.m2), _lib.ptr(self.t), _lib.ptr(self.trainable), float(lr),
                   float(b1), float(b2), float(eps), _lib.stream())
+
+    def capture_adam_step(self, lr=0.05, b1=0.9, b2=0.999, eps=1e-8):
+        """Capture one adam_step into a HIP graph (torch.cuda.CUDAGraph).
+        Every buffer the step touches is allocated beforehand by an eager
+        warm-up step, whose effect on the optimiser state is undone."""
+        from . import instrument
+        saved = [x.clone() for x in (self.u, self.m1, self.m2, self.t)]
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            self.adam_step(lr, b1, b2, eps)
+        torch.cuda.current_stream().wait_stream(side)
+        torch.cuda.synchronize()
+        for x, s in zip((self.u, self.m1, self.m2, self.t), saved):
+            x.copy_(s)
+        g = torch.cuda.CUDAGraph()
+        prev = instrument._ACTIVE
+        instrument._ACTIVE = None  # no timing events inside the capture
+        try:
+            with torch.cuda.graph(g):
+                self.adam_step(lr, b1, b2, eps)
+        finally:
+            instrument._ACTIVE = prev
+        torch.cuda.synchronize()
+        return g

@@ -85,7 +85,7 @@ class manifold_GPR:
 def train_gp(data, train_ind=None, n_inducing_points=None, test_size=0.2, kernel=None,
              noise_variance=0.001, kernel_lengthscale=None, kernel_variance=None, epochs=1000,
              positivity_constraint=1e-2, seed=0, *, reference_semantics=True, optimizer="lbfgs",
-             lr=0.05, form="auto", precision="fp64", verbose=True):
+             lr=0.05, form="auto", precision="fp64", verbose=True, use_graph=True):
     """Fit the vector-field GP (main.py:11-84)."""
     if train_ind is None:
         train_ind = np.arange(data.n)
@@ -131,8 +131,13 @@ def train_gp(data, train_ind=None, n_inducing_points=None, test_size=0.2, kernel
         eng.fit_lbfgs(maxiter=epochs, disp=verbose)
     elif optimizer == "adam":
         eng.adam_setup()
-        for _ in range(int(epochs)):
-            eng.adam_step(lr=lr)
+        if use_graph and epochs > 1:
+            g = eng.capture_adam_step(lr=lr)
+            for _ in range(int(epochs)):
+                g.replay()
+        else:
+            for _ in range(int(epochs)):
+                eng.adam_step(lr=lr)
         torch.cuda.synchronize()
     else:
         raise ValueError("optimizer must be 'lbfgs' or 'adam'")

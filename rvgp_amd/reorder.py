@@ -1,0 +1,65 @@
+"""Locality renumbering of graph nodes for the block SpMM.
+
+Points arrive in arbitrary order (the synthetic generators draw them
+uniformly at random), so the block columns of a block row are scattered over
+the whole operator and every gathered X row misses the caches.  Before the
+eigensolve the nodes are renumbered along a Morton (Z-order) curve of their
+leading principal coordinates: neighbours on the manifold get nearby
+indices, consecutive block rows share most of their neighbours, and the
+X rows a workgroup gathers stay resident in L2.
+
+The permuted operator P A P^T has the same spectrum; eigenvector block rows
+are mapped back with the inverse permutation.  Node identity is never
+renumbered anywhere else (the heap neighbourhoods, whose tie-breaking
+depends on node order, run on the original numbering).
+"""
+import torch
+
+
+def morton_perm(X, max_dims=6):
+    """Permutation (new position -> old node) sorting nodes by the Morton
+    code of their top-P principal coordinates (P = min(D, max_dims))."""
+    Xc = X - X.mean(0, keepdim=True)
+    C = Xc.T @ Xc
+    w, V = torch.linalg.eigh(C)
+    P = min(X.shape[1], max_dims)
+    Y = Xc @ V[:, -P:]
+    bits = min(20, 62 // P)
+    lo = Y.min(0).values
+    span = (Y.max(0).values - lo).clamp_min(1e-300)
+    q = ((Y - lo) / span * ((1 << bits) - 1)).round().to(torch.int64)
+    key = torch.zeros(X.shape[0], dtype=torch.int64, device=X.device)
+    for b in range(bits - 1, -1, -1):
+        for d in range(P):
+            key = (key << 1) | ((q[:, d] >> b) & 1)
+    return torch.sort(key, stable=True).indices
+
+
+def permute_bsr(A, perm):
+    """B = P A P^T for a BSR operator; perm[new] = old block row."""
+    from .sparse import BSR
+    dev = A.rowptr.device
+    nb, b = A.nb, A.b
+    inv = torch.empty_like(perm)
+    inv[perm] = torch.arange(nb, device=dev, dtype=perm.dtype)
+    rp = A.rowptr.long()
+    lens = rp[1:] - rp[:-1]
+    rows = torch.repeat_interleave(torch.arange(nb, device=dev), lens)
+    new_r = inv[rows]
+    new_c = inv[A.cols.long()]
+    order = torch.sort(new_r * nb + new_c, stable=True).indices
+    new_lens = lens[perm]
+    new_rp = torch.zeros(nb + 1, dtype=torch.int64, device=dev)
+    new_rp[1:] = torch.cumsum(new_lens, 0)
+    cols = new_c[order].to(torch.int32)
+    v64 = A.vals64.reshape(-1, b * b)[order].reshape(-1).contiguous()
+    v32 = None if A.vals32 is None else A.vals32.reshape(-1, b * b)[order].reshape(-1).contiguous()
+    return BSR(new_rp.to(torch.int32), cols, v64, v32, b)
+
+
+def unpermute_rows(V, perm, b):
+    """Rows of V in permuted block numbering -> original numbering."""
+    nb = perm.numel()
+    out = torch.empty_like(V)
+    out.view(nb, b, -1)[perm] = V.view(nb, b, -1)
+    return out

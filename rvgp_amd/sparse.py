@@ -7,7 +7,7 @@ the symmetrised graph's CSR), and the b x b blocks row-major in f64 and f32.
 import numpy as np
 import torch
 
-from . import _lib
+from . import _lib, instrument
 
 
 class BSR:
@@ -68,8 +68,28 @@ def spmm(A, X, Y=None, alpha=1.0, beta=0.0, Z=None, gamma=0.0, acc=None, delta=0
     A2 = None if acc is None else (acc.reshape(-1, 1) if acc.dim() == 1 else acc)
     name = "rvgp_bsr_spmm_f64" if X2.dtype == torch.float64 else "rvgp_bsr_spmm_f32"
     vals = A.vals(X2.dtype)
+    ev = instrument.begin()
     _lib.call(name, A.nb, A.b, _lib.ptr(A.rowptr), _lib.ptr(A.cols), _lib.ptr(vals),
               _lib.ptr(X2), X2.stride(0), r, float(alpha), float(beta), _lib.ptr(Z2),
               Z2.stride(0) if Z2 is not None else r, float(gamma), _lib.ptr(Y2), Y2.stride(0),
               _lib.ptr(A2), A2.stride(0) if A2 is not None else r, float(delta), _lib.stream())
+    if ev is not None:
+        s = X2.element_size()
+        nbytes = spmm_bytes(A, r, s, Z is not None and gamma != 0.0, acc is not None)
+        kind = "spmm_narrow" if r <= 4 else "spmm_wide"
+        instrument.end(ev, kind, (str(X2.dtype).replace("torch.", ""), A.b), nbytes,
+                       2.0 * A.nnzb * A.b * A.b * r)
     return Y
+
+
+def spmm_bytes(A, r, s, with_z=False, with_acc=False):
+    """Algorithmic HBM bytes of one fused SpMM launch: block values and column
+    indices once, row pointer once, X read once, Y written once, plus Z read
+    and acc read + written when the epilogue uses them."""
+    dense_rows = A.nb * A.b * r * s
+    b = A.nnzb * (A.b * A.b * s + 4) + 4 * (A.nb + 1) + 2 * dense_rows
+    if with_z:
+        b += dense_rows
+    if with_acc:
+        b += 2 * dense_rows
+    return b

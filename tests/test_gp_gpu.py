@@ -148,3 +148,23 @@ def test_lbfgs_reaches_oracle_optimum():
     th_ref, res_ref = cpu.fit_lbfgs(Phi, y, lam, 600.0, maxiter=200)
     assert abs(res.fun - res_ref.fun) < 1e-6 * abs(res_ref.fun)
     np.testing.assert_allclose(th, th_ref, rtol=1e-3)
+
+
+@pytest.mark.parametrize("form", ["weight", "function"])
+def test_adam_graph_replays_match_eager(form):
+    """Every replay of the captured optimiser step equals the eager step,
+    with eager work and synchronisations between the replays."""
+    import torch
+    Phi, y, lam = problem(M_nodes=120, k=20, seed=19)
+    a = engine(Phi, y, lam, 500.0, form)
+    b = engine(Phi, y, lam, 500.0, form)
+    a.adam_setup()
+    b.adam_setup()
+    g = b.capture_adam_step(lr=0.05)
+    for step in range(12):
+        a.adam_step(lr=0.05)
+        g.replay()
+        torch.cuda.synchronize()
+        _ = (torch.ones(1000, device="cuda") * step).sum().item()  # unrelated eager work
+        np.testing.assert_allclose(b.u.cpu().numpy(), a.u.cpu().numpy(), rtol=1e-12, atol=1e-14)
+        np.testing.assert_allclose(b.out.cpu().numpy(), a.out.cpu().numpy(), rtol=1e-11, atol=1e-12)

@@ -73,7 +73,7 @@ def run_fit(X, V, train, precision, args, timings):
                                 timings=timings, eig_verbose=args.verbose)
     t1 = time.perf_counter()
     gp = RVGP.fit(d, train_ind=train, epochs=args.adam_steps, optimizer="adam", lr=0.05,
-                  precision=precision, verbose=False)
+                  precision=precision, verbose=False, timings=timings)
     torch.cuda.synchronize()
     t2 = time.perf_counter()
     timings["data_object"] = t1 - t0
@@ -110,12 +110,13 @@ def cpu_baseline(args):
     GPR) on the same generator at n = args.cpu_n: data object timed in full,
     NLML + gradient evaluations timed and scaled to the fit's Adam steps."""
     import numpy as np
+    from threadpoolctl import threadpool_info, threadpool_limits
     from oracle import cpu
-    try:
-        from threadpoolctl import threadpool_info
-        cores = max([t.get("num_threads", 1) for t in threadpool_info()] + [1])
-    except Exception:
-        cores = os.cpu_count()
+    # BLAS pools sized to the CPU share this job runs with, not to the host's
+    # core count (an oversubscribed pool is many times slower)
+    share = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    threadpool_limits(limits=min(16, share))
+    cores = max([t.get("num_threads", 1) for t in threadpool_info()] + [1])
     n = args.cpu_n
     X, a, Q6 = flat_torus3(n, seed=0)
     V = torus_field(a, Q6)
@@ -125,16 +126,19 @@ def cpu_baseline(args):
     t0 = time.perf_counter()
     o = cpu.Data(X, n_neighbors=args.k_nn, n_eigenpairs=k, allow_rank_deficient_k=True)
     t_data = time.perf_counter() - t0
+    log(f"[bench] CPU oracle data object n={n}: {t_data:.2f}s ({cores} BLAS threads)")
     tr, _ = cpu.train_test_split_indices(len(train), 0.2, 0)
     nodes = train[tr]
     Phi = o.evecs_Lc.reshape(n, -1)[nodes].reshape(-1, k)
     y = V[nodes].reshape(-1, 1)
     evals = 2
     t0 = time.perf_counter()
-    for _ in range(evals):
+    for e in range(evals):
         S, dS = cpu.spectral_filter(o.evals_Lc, 1.5, 5.0, 1.0, n * X.shape[1])
         cpu.gpr_nlml(Phi, y, S, 1.0)
         cpu.gpr_grad(Phi, y, S, 1.0)
+        log(f"[bench] CPU oracle NLML+gradient {e + 1}/{evals}: "
+            f"{time.perf_counter() - t0:.2f}s so far")
     t_eval = (time.perf_counter() - t0) / evals
     total = t_data + args.adam_steps * t_eval
     return {"value": total, "unit": "s per fit", "cores": int(cores), "kind": "port",
@@ -157,6 +161,55 @@ def gpu_same_sample(args):
     return t
 
 
+def roofline_only(args):
+    """Launch the dominant kernel alone at the headline shape: the FP32 block
+    SpMM (b = 3) of the Lc eigensolve on the Morton-renumbered operator, with
+    the Chebyshev-step epilogue (Z read) at the starting block width
+    r = k + guard rounded up to a multiple of 8,
+    then a plain 1 GiB device copy (the reference for scaling PMC byte
+    counters).  Used under rocprofv3 --pmc; prints one JSON line."""
+    import torch
+    from rvgp_amd import _lib, geometry
+    from rvgp_amd.ptu_dijkstra import connections_device, tangent_frames_device
+    from rvgp_amd.reorder import morton_perm, permute_bsr
+    from rvgp_amd.sparse import BSR, spmm, spmm_bytes
+    _lib.lib()
+    X, _, _ = flat_torus3(args.n, seed=0)
+    Xd = torch.as_tensor(X, device="cuda")
+    G = geometry.manifold_graph(Xd, n_neighbors=args.k_nn)
+    K = int(args.k_nn * 1.5)
+    D = X.shape[1]
+    T, _ = tangent_frames_device(Xd, G, K, min(D, K + 1), min(D, K + 1), min(D, K))
+    c = connections_device(T[:, :, :3].contiguous(), G, 3)
+    A = permute_bsr(BSR(G.rowptr, G.cols, c["lc64"].reshape(-1), c["lc32"].reshape(-1), 3),
+                    morton_perm(Xd))
+    r = -(-(args.k_eig + max(8, int(math.ceil(0.15 * args.k_eig)))) // 8) * 8
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(0)
+    Xm = torch.randn((A.shape[0], r), generator=gen, device="cuda", dtype=torch.float32)
+    Z = torch.randn_like(Xm)
+    for _ in range(2):
+        spmm(A, Xm, Y=Z, alpha=0.5, beta=0.1, Z=Z, gamma=-0.3)
+    torch.cuda.synchronize()
+    reps = 10
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        spmm(A, Xm, Y=Z, alpha=0.5, beta=0.1, Z=Z, gamma=-0.3)
+    e1.record()
+    torch.cuda.synchronize()
+    us = 1e3 * e0.elapsed_time(e1) / reps
+    nbytes = spmm_bytes(A, r, 4, with_z=True)
+    src = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        torch.mul(src, 1.0, out=dst)  # an elementwise kernel: reads 1 GiB, writes 1 GiB
+    torch.cuda.synchronize()
+    print(json.dumps({"kernel": "spmm_vec<float,3,4,*>", "r": r, "nb": A.nb, "nnzb": A.nnzb,
+                      "launches": reps + 2, "avg_us": us, "alg_bytes": nbytes,
+                      "GBps": nbytes / (us * 1e3), "copy_bytes_each_way": 1 << 30}))
+
+
 # ---------------------------------------------------------------- main
 def main():
     p = argparse.ArgumentParser()
@@ -169,14 +222,18 @@ def main():
     p.add_argument("--adam-steps", dest="adam_steps", type=int, default=200)
     p.add_argument("--skip-fp64", action="store_true")
     p.add_argument("--skip-cpu", action="store_true")
-    p.add_argument("--cpu-n", dest="cpu_n", type=int, default=800)
+    p.add_argument("--cpu-n", dest="cpu_n", type=int, default=1000)
     p.add_argument("--cpu-k-eig", dest="cpu_k_eig", type=int, default=50)
     p.add_argument("--verbose", action="store_true")
+    p.add_argument("--roofline-only", dest="roofline_only", action="store_true")
     args = p.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
               "design is described in DESIGN.md but not built.", file=sys.stderr)
         sys.exit(2)
+    if args.roofline_only:
+        roofline_only(args)
+        return
 
     import numpy as np
     import torch
@@ -237,7 +294,8 @@ def main():
     # roofline of the dominant kernel: FP32 block SpMM (b = dim_man) of the
     # Lc eigensolve, measured over every launch of that kernel in the timed fit
     dm = d_mx.dim_man
-    key = f"spmm_wide[float32,b{dm}]"
+    cands = [f"spmm_vec[float32,b{dm}]", f"spmm_wide[float32,b{dm}]"]
+    key = max(cands, key=lambda c: kstats.get(c, {}).get("total_ms", 0.0))
     ks = kstats.get(key, {"count": 0, "avg_us": 0.0, "GBps": 0.0})
     roof = {"kernel": key, "bound": "hbm", "achieved": round(ks["GBps"], 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(ks["GBps"] / HBM_PEAK_GBS, 4),

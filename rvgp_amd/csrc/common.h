@@ -55,6 +55,17 @@ __device__ __forceinline__ T wave_max(T v) {
     return v;
 }
 
+// Broadcast lane `l` of v (l wave-uniform) -- v_readlane into an SGPR.
+__device__ __forceinline__ float bcast(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ double bcast(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 // Direct squared distance with a fixed operation order:
 //   acc = fma(t_k, t_k, acc), t_k = a_k - b_k, k = 0..D-1.
 // The CPU oracle uses the same sequence, so results agree bit for bit.

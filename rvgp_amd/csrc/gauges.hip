@@ -7,7 +7,9 @@
 // the reference does, and reduced to the smaller of its two symmetric Gram
 // matrices (D x D covariance when D <= K+1, else (K+1) x (K+1)).  A cyclic
 // Jacobi eigensolve with the round-robin pair ordering rotates m/2 disjoint
-// pairs per step, the lanes splitting the row / column / vector updates.
+// pairs per step; the two-sided update J^T A J of one step is one pass over
+// independent 2 x 2 blocks, the lanes splitting the blocks and the vector
+// update.
 // Singular values are the square roots of the eigenvalues; the left singular
 // vectors are the eigenvectors (covariance case) or G v / s (Gram case).
 // Singular-vector signs are free, as they are for LAPACK's dgesvd.
@@ -29,8 +31,9 @@ __global__ __launch_bounds__(64) void gauges_kernel(const double* __restrict__ x
     const int m2 = m + (m & 1);
     double* G = smem;                  // D x Kp1
     double* A = G + D * Kp1;           // m2 x m2
-    double* V = A + m2 * m2;           // m2 x m2
-    double* cs = V + m2 * m2;          // m2/2 cosines
+    const int lda = m2 + 1;            // padded row stride of A and V
+    double* V = A + m2 * lda;          // m2 x m2
+    double* cs = V + m2 * lda;         // m2/2 cosines
     double* sn = cs + m2 / 2;          // m2/2 sines
     int* pp = reinterpret_cast<int*>(sn + m2 / 2);  // m2/2
     int* qq = pp + m2 / 2;                          // m2/2
@@ -61,22 +64,29 @@ __global__ __launch_bounds__(64) void gauges_kernel(const double* __restrict__ x
                 for (int p = 0; p < D; ++p) s = __fma_rn(G[p * Kp1 + a], G[p * Kp1 + b], s);
             }
         }
-        A[e] = s;
-        V[e] = (a == b) ? 1.0 : 0.0;
+        A[a * lda + b] = s;
+        V[a * lda + b] = (a == b) ? 1.0 : 0.0;
     }
     __syncthreads();
 
     const int half = m2 / 2;
+    // t / half and t / m2 for t < 4096 by multiply-shift (exact: the rounding
+    // error t (ceil(2^20/d) - 2^20/d) / 2^20 < 1/64 <= 1/d, and t * r < 2^32)
+    const unsigned rh = ((1u << 20) + half - 1) / half, rm = ((1u << 20) + m2 - 1) / m2;
     for (int sweep = 0; sweep < 40; ++sweep) {
         double off = 0.0, dia = 0.0;
-        for (int e = lane; e < m2 * m2; e += 64) {
-            int a = e / m2, b = e - a * m2;
-            double v = A[e] * A[e];
-            if (a == b) dia += v; else off += v;
+        if (lane < m2) {
+            for (int a = 0; a < m2; ++a) {
+                const double v = A[a * lda + lane];
+                if (a == lane) dia += v * v; else off += v * v;
+            }
         }
         off = wave_sum(off);
         dia = wave_sum(dia);
-        if (!(off > 1e-32 * dia)) break;  // also stops on an all-zero matrix
+        // stop once the off-diagonal mass is at the rounding level of the
+        // diagonal (|off| <= 1e-14 |diag|); Jacobi converges quadratically, so
+        // the last sweep before the test passes already lands far below it
+        if (!(off > 1e-28 * dia)) break;  // also stops on an all-zero matrix
         for (int step = 0; step < m2 - 1; ++step) {
             if (lane < half) {
                 // round robin: position 0 fixed, positions 1..m2-1 rotate
@@ -84,10 +94,10 @@ __global__ __launch_bounds__(64) void gauges_kernel(const double* __restrict__ x
                 int bpos = m2 - 1 - lane;
                 int b = 1 + (bpos - 1 + step) % (m2 - 1);
                 int p = a < b ? a : b, q = a < b ? b : a;
-                double apq = A[p * m2 + q];
+                double apq = A[p * lda + q];
                 double c = 1.0, s = 0.0;
                 if (apq != 0.0) {
-                    double tau = (A[q * m2 + q] - A[p * m2 + p]) / (2.0 * apq);
+                    double tau = (A[q * lda + q] - A[p * lda + p]) / (2.0 * apq);
                     double t;
                     if (fabs(tau) > 1e150) t = 0.5 / tau;
                     else t = (tau >= 0.0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1.0 + tau * tau));
@@ -100,35 +110,41 @@ __global__ __launch_bounds__(64) void gauges_kernel(const double* __restrict__ x
                 sn[lane] = s;
             }
             __syncthreads();
-            for (int e = lane; e < half * m2; e += 64) {  // A <- A J, V <- V J
-                int k = e / m2, r = e - k * m2;
-                int p = pp[k], q = qq[k];
-                double c = cs[k], s = sn[k];
-                double arp = A[r * m2 + p], arq = A[r * m2 + q];
-                A[r * m2 + p] = c * arp - s * arq;
-                A[r * m2 + q] = s * arp + c * arq;
-                double vrp = V[r * m2 + p], vrq = V[r * m2 + q];
-                V[r * m2 + p] = c * vrp - s * vrq;
-                V[r * m2 + q] = s * vrp + c * vrq;
+            // A <- J^T A J in one pass: the pairs partition the indices, so
+            // each entry lies in exactly one 2 x 2 block (rows of pair k,
+            // columns of pair k2) and the blocks update independently.
+            for (int t = lane; t < half * half; t += 64) {
+                const int k = (int)(((unsigned)t * rh) >> 20), k2 = t - k * half;
+                const int p = pp[k], q = qq[k], p2 = pp[k2], q2 = qq[k2];
+                const double c = cs[k], s = sn[k], c2 = cs[k2], s2 = sn[k2];
+                const double a00 = A[p * lda + p2], a01 = A[p * lda + q2];
+                const double a10 = A[q * lda + p2], a11 = A[q * lda + q2];
+                // rows: [p; q] <- [c -s; s c]^T ... applied as J^T on the left
+                const double b00 = c * a00 - s * a10, b01 = c * a01 - s * a11;
+                const double b10 = s * a00 + c * a10, b11 = s * a01 + c * a11;
+                // columns: [p2 q2] <- [p2 q2] [c2 s2; -s2 c2]
+                A[p * lda + p2] = c2 * b00 - s2 * b01;
+                A[p * lda + q2] = s2 * b00 + c2 * b01;
+                A[q * lda + p2] = c2 * b10 - s2 * b11;
+                A[q * lda + q2] = s2 * b10 + c2 * b11;
             }
-            __syncthreads();
-            for (int e = lane; e < half * m2; e += 64) {  // A <- J^T A
-                int k = e / m2, r = e - k * m2;
-                int p = pp[k], q = qq[k];
-                double c = cs[k], s = sn[k];
-                double apr = A[p * m2 + r], aqr = A[q * m2 + r];
-                A[p * m2 + r] = c * apr - s * aqr;
-                A[q * m2 + r] = s * apr + c * aqr;
+            for (int t = lane; t < half * m2; t += 64) {  // V <- V J
+                const int k = (int)(((unsigned)t * rm) >> 20), r = t - k * m2;
+                const int p = pp[k], q = qq[k];
+                const double c = cs[k], s = sn[k];
+                const double vrp = V[r * lda + p], vrq = V[r * lda + q];
+                V[r * lda + p] = c * vrp - s * vrq;
+                V[r * lda + q] = s * vrp + c * vrq;
             }
             __syncthreads();
         }
     }
     // eigenvalues descending (ties: lower index first)
     for (int a = lane; a < m; a += 64) {
-        double la = A[a * m2 + a];
+        double la = A[a * lda + a];
         int rank = 0;
         for (int b = 0; b < m; ++b) {
-            double lb = A[b * m2 + b];
+            double lb = A[b * lda + b];
             rank += (lb > la) || (lb == la && b < a);
         }
         order[rank] = a;
@@ -136,7 +152,7 @@ __global__ __launch_bounds__(64) void gauges_kernel(const double* __restrict__ x
     __syncthreads();
     int bad = 0;
     for (int r = lane; r < m; r += 64) {
-        double lam = A[order[r] * m2 + order[r]];
+        double lam = A[order[r] * lda + order[r]];
         double s = sqrt(lam > 0.0 ? lam : 0.0);
         if (r < nsv) sigma[(size_t)i * nsv + r] = s;
         if (r < n_check && s < 1e-10) bad = 1;
@@ -146,12 +162,12 @@ __global__ __launch_bounds__(64) void gauges_kernel(const double* __restrict__ x
         int c = order[r];
         double u;
         if (cov) {
-            u = V[p * m2 + c];
+            u = V[p * lda + c];
         } else {
-            double lam = A[c * m2 + c];
+            double lam = A[c * lda + c];
             double s = sqrt(lam > 0.0 ? lam : 0.0);
             double acc = 0.0;
-            for (int q = 0; q < Kp1; ++q) acc = __fma_rn(G[p * Kp1 + q], V[q * m2 + c], acc);
+            for (int q = 0; q < Kp1; ++q) acc = __fma_rn(G[p * Kp1 + q], V[q * lda + c], acc);
             u = s > 0.0 ? acc / s : 0.0;
         }
         tangents[(size_t)i * D * ncols + e] = u;
@@ -162,7 +178,7 @@ __global__ __launch_bounds__(64) void gauges_kernel(const double* __restrict__ x
 static size_t gauges_smem(int D, int Kp1) {
     int m = D <= Kp1 ? D : Kp1;
     int m2 = m + (m & 1);
-    size_t dbl = (size_t)D * Kp1 + 2 * (size_t)m2 * m2 + (size_t)m2;
+    size_t dbl = (size_t)D * Kp1 + 2 * (size_t)m2 * (m2 + 1) + (size_t)m2;
     size_t ints = 2 * (size_t)(m2 / 2) + Kp1 + m;
     return dbl * sizeof(double) + ints * sizeof(int);
 }

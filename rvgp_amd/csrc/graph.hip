@@ -1,11 +1,18 @@
 // a2  kNN graph (RVGP/geometry.py:100-123) and its symmetrised CSR
 // (RVGP/lib/ptu_dijkstra.pyx:84-103).
 //
-// kNN: brute force, one query per lane, candidate tiles staged in LDS so a
-// wave reads each candidate coordinate as an LDS broadcast; the running top-k
+// kNN: brute force.  A workgroup owns 64 queries (one per lane) and its four
+// waves split the candidate range into four contiguous slices, so n = 1e5
+// queries still put ~6 waves on every SIMD.  Each wave stages its candidate
+// tile in LDS and reads each coordinate as a broadcast; the running top-k
 // lives in registers as an unrolled sorted list (compare-and-shift network,
-// no dynamic register indexing).  Candidates arrive in increasing index
-// order, so a strict "<" keeps the lower index among equal distances.
+// no dynamic register indexing) with the current k-th distance cached in a
+// register, so a rejected candidate costs one compare.  Coordinates past D
+// are zero in both the query and the tile: fma(0, 0, acc) == acc, so the
+// padded sum is bit-identical to the D-term one.  Within a slice candidates
+// arrive in increasing index order and a strict "<" keeps the lower index
+// among equal distances; the slices are then merged into wave 0 in slice
+// order with the same insertion, which keeps that order across slices.
 //
 // Symmetrisation: degree count (atomics) -> exclusive scan -> scatter of
 // forward + reverse edges + self loop -> per-row sort + de-duplicate ->
@@ -15,14 +22,41 @@
 
 namespace rvgp {
 
-constexpr int kKnnThreads = 256;
-constexpr int kKnnTile = 64;
+constexpr int kKnnSlices = 4;             // waves per workgroup = candidate slices
+constexpr int kKnnThreads = 64 * kKnnSlices;
+constexpr int kKnnTileDoubles = 1024;     // LDS doubles per wave tile
+
+// The running list is right-aligned in KMAX slots: slots [0, KMAX-k) hold a
+// -inf sentinel that never moves, slots [KMAX-k, KMAX) hold the k best so far
+// in ascending order (+inf while empty), so the k-th distance is always
+// bd[KMAX-1].  Insert (d, j), d < bd[KMAX-1]: entries <= d stay in front,
+// larger ones shift back by one and the last one drops out.
+template <int KMAX>
+__device__ __forceinline__ void knn_insert(double (&bd)[KMAX], int (&bi)[KMAX], double d, int j) {
+#pragma unroll
+    for (int p = KMAX - 1; p > 0; --p) {
+        const bool shift = bd[p - 1] > d;
+        const bool here = !shift && bd[p] > d;
+        bi[p] = shift ? bi[p - 1] : (here ? j : bi[p]);
+        bd[p] = shift ? bd[p - 1] : (here ? d : bd[p]);
+    }
+    if (bd[0] > d) {
+        bd[0] = d;
+        bi[0] = j;
+    }
+}
 
 template <int DMAX, int KMAX>
 __global__ __launch_bounds__(kKnnThreads) void knn_kernel(const double* __restrict__ x, int n,
                                                           int D, int k, int* __restrict__ nbr) {
-    __shared__ double tile[kKnnTile * DMAX];
-    const int i = blockIdx.x * kKnnThreads + threadIdx.x;
+    constexpr int TILE = kKnnTileDoubles / DMAX;  // candidates per tile
+    constexpr int TILE_B = kKnnSlices * kKnnTileDoubles * 8;
+    constexpr int MERGE_B = 64 * KMAX * 12;
+    __shared__ __align__(16) unsigned char lds[TILE_B > MERGE_B ? TILE_B : MERGE_B];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int i = blockIdx.x * 64 + lane;
+    double* tile = reinterpret_cast<double*>(lds) + w * kKnnTileDoubles;
+
     double q[DMAX];
 #pragma unroll
     for (int c = 0; c < DMAX; ++c) q[c] = (i < n && c < D) ? x[(size_t)i * D + c] : 0.0;
@@ -30,53 +64,59 @@ __global__ __launch_bounds__(kKnnThreads) void knn_kernel(const double* __restri
     int bi[KMAX];
 #pragma unroll
     for (int p = 0; p < KMAX; ++p) {
-        bd[p] = __longlong_as_double(0x7ff0000000000000LL);  // +inf
+        // -inf sentinels in front, +inf (empty) in the k live slots
+        bd[p] = __longlong_as_double(p < KMAX - k ? 0xfff0000000000000LL : 0x7ff0000000000000LL);
         bi[p] = -1;
     }
-    for (int j0 = 0; j0 < n; j0 += kKnnTile) {
+
+    // Slice w = candidates [s0, s1); every wave runs the same trip count.
+    const int per = ((n + kKnnSlices - 1) / kKnnSlices + TILE - 1) / TILE * TILE;
+    const int s0 = w * per, s1 = min(n, s0 + per);
+    for (int t = 0; t < per; t += TILE) {
+        const int j0 = s0 + t;
         __syncthreads();
-        for (int e = threadIdx.x; e < kKnnTile * D; e += kKnnThreads) {
-            int jj = e / D, c = e - jj * D;
-            int j = j0 + jj;
-            tile[jj * DMAX + c] = (j < n) ? x[(size_t)j * D + c] : 0.0;
+        for (int e = lane; e < TILE * DMAX; e += 64) {
+            const int jj = e / DMAX, c = e % DMAX, j = j0 + jj;
+            tile[e] = (j < s1 && c < D) ? x[(size_t)j * D + c] : 0.0;
         }
         __syncthreads();
-        const int jend = min(kKnnTile, n - j0);
+        const int jend = min(TILE, s1 - j0);
         for (int jj = 0; jj < jend; ++jj) {
             const int j = j0 + jj;
             double acc = 0.0;
 #pragma unroll
-            for (int c = 0; c < DMAX; ++c)
-                if (c < D) {
-                    double t = __dsub_rn(q[c], tile[jj * DMAX + c]);
-                    acc = __fma_rn(t, t, acc);
-                }
-            // worst kept entry is bd[k-1]; select it without dynamic indexing
-            double worst = bd[0];
+            for (int c = 0; c < DMAX; ++c) {
+                const double tt = __dsub_rn(q[c], tile[jj * DMAX + c]);
+                acc = __fma_rn(tt, tt, acc);
+            }
+            if (j != i && acc < bd[KMAX - 1]) knn_insert<KMAX>(bd, bi, acc, j);
+        }
+    }
+
+    // Merge slices 1..3 into wave 0, in slice order.
+    double* md = reinterpret_cast<double*>(lds);
+    int* mi = reinterpret_cast<int*>(lds + 64 * KMAX * 8);
+    for (int src = 1; src < kKnnSlices; ++src) {
+        __syncthreads();
+        if (w == src) {
 #pragma unroll
-            for (int p = 1; p < KMAX; ++p)
-                if (p == k - 1) worst = bd[p];
-            if (j != i && acc < worst) {
-#pragma unroll
-                for (int p = KMAX - 1; p >= 0; --p) {
-                    if (p < k && bd[p] > acc) {
-                        double prev = (p > 0) ? bd[p > 0 ? p - 1 : 0] : -1.0;
-                        if (p > 0 && prev > acc) {
-                            bd[p] = prev;
-                            bi[p] = bi[p > 0 ? p - 1 : 0];
-                        } else {
-                            bd[p] = acc;
-                            bi[p] = j;
-                        }
-                    }
-                }
+            for (int p = 0; p < KMAX; ++p) {
+                md[p * 64 + lane] = bd[p];
+                mi[p * 64 + lane] = bi[p];
+            }
+        }
+        __syncthreads();
+        if (w == 0) {
+            for (int p = KMAX - k; p < KMAX; ++p) {
+                const double d = md[p * 64 + lane];
+                if (d < bd[KMAX - 1]) knn_insert<KMAX>(bd, bi, d, mi[p * 64 + lane]);
             }
         }
     }
-    if (i < n) {
+    if (w == 0 && i < n) {
 #pragma unroll
         for (int p = 0; p < KMAX; ++p)
-            if (p < k) nbr[(size_t)i * k + p] = bi[p];
+            if (p >= KMAX - k) nbr[(size_t)i * k + p - (KMAX - k)] = bi[p];
     }
 }
 
@@ -172,6 +212,8 @@ template <int DMAX>
 static void launch_knn(dim3 g, hipStream_t s, const double* x, int n, int D, int k, int* nbr) {
     if (k <= 16)
         hipLaunchKernelGGL((knn_kernel<DMAX, 16>), g, dim3(kKnnThreads), 0, s, x, n, D, k, nbr);
+    else if (k <= 24)
+        hipLaunchKernelGGL((knn_kernel<DMAX, 24>), g, dim3(kKnnThreads), 0, s, x, n, D, k, nbr);
     else if (k <= 32)
         hipLaunchKernelGGL((knn_kernel<DMAX, 32>), g, dim3(kKnnThreads), 0, s, x, n, D, k, nbr);
     else
@@ -182,7 +224,7 @@ extern "C" int rvgp_knn(const double* x, int n, int D, int k, int* nbr, void* st
     RVGP_CHECK_ARG(k > 0 && k < n, "rvgp_knn: need 0 < k < n (k=%d, n=%d)", k, n);
     RVGP_CHECK_ARG(k <= 64, "rvgp_knn: k <= 64 supported");
     RVGP_CHECK_ARG(D >= 1 && D <= 64, "rvgp_knn: 1 <= D <= 64 supported");
-    dim3 g(cdiv(n, kKnnThreads));
+    dim3 g(cdiv(n, 64));  // 64 queries per workgroup
     hipStream_t s = as_stream(stream);
     if (D <= 4) launch_knn<4>(g, s, x, n, D, k, nbr);
     else if (D <= 8) launch_knn<8>(g, s, x, n, D, k, nbr);

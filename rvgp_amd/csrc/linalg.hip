@@ -3,9 +3,9 @@
 // RVGP/main.py:89-91 and 110-111).
 //
 // potrf: right-looking, NB = 64.  Per block column: (1) the diagonal block
-// is factorised by one workgroup in LDS, (2) the panel below is solved
-// against it (one workgroup per 64 rows, L11 in LDS), (3) the trailing
-// matrix gets A22 -= A21 A21^T on the matrix cores (rvgp_gemm).
+// is factorised by one wavefront with its rows in registers, (2) the panel
+// below is solved against it (one row per thread, L11 in LDS), (3) the
+// trailing matrix gets A22 -= A21 A21^T on the matrix cores (rvgp_gemm).
 // trsm: blocked substitution; the off-diagonal updates are GEMMs, the
 // diagonal-block solves one lane per right-hand-side column.
 #include "common.h"
@@ -19,110 +19,93 @@ namespace rvgp {
 
 constexpr int NB = 64;
 
+// Diagonal block, one wavefront: lane r holds row r of the block in
+// registers; the column loop is fully unrolled, so every register index is a
+// constant and the entries other lanes need (pivot, scaled column) arrive by
+// v_readlane.  No LDS and no barriers.  Rows past nb are identity rows.
 template <typename T>
-__global__ __launch_bounds__(256) void potrf_diag(T* __restrict__ A, int lda, int j0, int nb,
-                                                  int* __restrict__ status) {
-    __shared__ T L[NB][NB + 1];
-    const int tid = threadIdx.x;
-    for (int e = tid; e < nb * nb; e += 256) {
-        int r = e / nb, c = e - r * nb;
-        L[r][c] = A[(size_t)(j0 + r) * lda + j0 + c];
-    }
-    __syncthreads();
-    __shared__ int bad;
-    if (tid == 0) bad = 0;
-    __syncthreads();
-    for (int c = 0; c < nb; ++c) {
-        const T piv = L[c][c];
-        if (!(piv > T(0))) {
-            if (tid == 0) bad = 1;
-        }
-        __syncthreads();
-        if (bad) break;
+__global__ __launch_bounds__(64) void potrf_diag(T* __restrict__ A, int lda, int j0, int nb,
+                                                 int* __restrict__ status) {
+    const int r = threadIdx.x;
+    T a[NB];
+#pragma unroll
+    for (int c = 0; c < NB; ++c)
+        a[c] = (r < nb && c <= r) ? A[(size_t)(j0 + r) * lda + j0 + c] : (c == r ? T(1) : T(0));
+    bool bad = false;
+#pragma unroll
+    for (int c = 0; c < NB; ++c) {
+        const T piv = bcast(a[c], c);
+        bad |= !(piv > T(0));
         const T lcc = sqrt(piv);
-        __syncthreads();
-        if (tid == 0) L[c][c] = lcc;
-        for (int r = c + 1 + tid; r < nb; r += 256) L[r][c] /= lcc;
-        __syncthreads();
-        const int rem = nb - c - 1;
-        for (int e = tid; e < rem * rem; e += 256) {
-            int r = c + 1 + e / rem, s = c + 1 + e % rem;
-            if (s <= r) L[r][s] -= L[r][c] * L[s][c];
+        if (r == c) a[c] = lcc;
+        if (r > c) a[c] = a[c] / lcc;
+#pragma unroll
+        for (int s = c + 1; s < NB; ++s) {
+            const T lsc = bcast(a[c], s);
+            if (s <= r) a[s] -= a[c] * lsc;
         }
-        __syncthreads();
     }
-    if (bad) {
-        if (tid == 0) atomicMax(status, RVGP_ST_NOT_SPD);
+    if (bad) {  // wave-uniform: every lane saw the same pivots
+        if (r == 0) atomicMax(status, RVGP_ST_NOT_SPD);
         return;
     }
-    for (int e = tid; e < nb * nb; e += 256) {
-        int r = e / nb, c = e - r * nb;
-        if (c <= r) A[(size_t)(j0 + r) * lda + j0 + c] = L[r][c];
+    if (r < nb) {
+#pragma unroll
+        for (int c = 0; c < NB; ++c)
+            if (c <= r) A[(size_t)(j0 + r) * lda + j0 + c] = a[c];
     }
 }
 
-// Rows i in [j0+nb, n): x L11^T = a  (forward substitution along the row).
+// Solve the nb x nb diagonal block L11 (at Lm[i0][i0], lower triangular) for
+// r right-hand sides: X = L11^-1 B (trans = 0) or L11^-T B (trans = 1), where
+// element (a, c) of B is B[a * rs + c * cs] -- rs = ldb, cs = 1 for the rows
+// [i0, i0+nb) of a row-major B, and rs = 1, cs = lda for the Cholesky panel,
+// whose rows are right-hand sides (x L11^T = a  <=>  L11 x^T = a^T).  One
+// right-hand side per thread, held in registers; L11 staged in LDS and read
+// as broadcasts.  Each dot product runs as four interleaved partial sums so
+// that the dependent FMA chain is a quarter as long.
 template <typename T>
-__global__ __launch_bounds__(64) void potrf_panel(T* __restrict__ A, int lda, int n, int j0,
-                                                  int nb, const int* __restrict__ status) {
-    if (*status) return;
-    __shared__ T L[NB][NB + 1];
-    __shared__ T X[64][NB + 1];
-    const int tid = threadIdx.x;
-    const int r0 = j0 + nb + blockIdx.x * 64;
-    for (int e = tid; e < nb * nb; e += 64) {
-        int r = e / nb, c = e - r * nb;
-        L[r][c] = A[(size_t)(j0 + r) * lda + j0 + c];
-    }
-    for (int e = tid; e < 64 * nb; e += 64) {
-        int r = e / nb, c = e - r * nb;
-        X[r][c] = (r0 + r < n) ? A[(size_t)(r0 + r) * lda + j0 + c] : T(0);
-    }
-    __syncthreads();
-    if (r0 + tid < n) {
-        for (int c = 0; c < nb; ++c) {
-            T s = X[tid][c];
-            for (int t = 0; t < c; ++t) s -= X[tid][t] * L[c][t];
-            X[tid][c] = s / L[c][c];
-        }
-    }
-    __syncthreads();
-    for (int e = tid; e < 64 * nb; e += 64) {
-        int r = e / nb, c = e - r * nb;
-        if (r0 + r < n) A[(size_t)(r0 + r) * lda + j0 + c] = X[r][c];
-    }
-}
-
-// Solve the nb x nb diagonal block for all r columns of B rows [i0, i0+nb).
-template <typename T>
-__global__ __launch_bounds__(64) void trsm_diag(int trans, const T* __restrict__ Lm, int lda,
-                                                int i0, int nb, int r, T* __restrict__ B,
-                                                int ldb) {
+__global__ __launch_bounds__(256) void trsm_diag(int trans, const T* __restrict__ Lm, int lda,
+                                                 int i0, int nb, int r, T* __restrict__ B,
+                                                 long long rs, long long cs,
+                                                 const int* __restrict__ status) {
+    if (status != nullptr && *status) return;
     __shared__ T L[NB][NB + 1];
     const int tid = threadIdx.x;
-    for (int e = tid; e < nb * nb; e += 64) {
-        int a = e / nb, b = e - a * nb;
-        L[a][b] = (b <= a) ? Lm[(size_t)(i0 + a) * lda + i0 + b] : T(0);
+    for (int e = tid; e < NB * NB; e += 256) {  // whole tile: zeros outside the block
+        int a = e / NB, b = e - a * NB;
+        L[a][b] = (a < nb && b <= a) ? Lm[(size_t)(i0 + a) * lda + i0 + b] : T(0);
     }
     __syncthreads();
-    const int c = blockIdx.x * 64 + tid;
+    const int c = blockIdx.x * 256 + tid;
     if (c >= r) return;
+    // The right-hand side lives in registers: every loop is unrolled over the
+    // fixed block size, so x[] is only ever indexed by constants.  Rows beyond
+    // nb carry x = 0, which leaves the solve unchanged.
+    T* bc = B + (size_t)c * cs;
     T x[NB];
-    for (int a = 0; a < nb; ++a) x[a] = B[(size_t)(i0 + a) * ldb + c];
+#pragma unroll
+    for (int a = 0; a < NB; ++a) x[a] = a < nb ? bc[(size_t)a * rs] : T(0);
     if (!trans) {
-        for (int a = 0; a < nb; ++a) {
-            T s = x[a];
-            for (int b = 0; b < a; ++b) s -= L[a][b] * x[b];
-            x[a] = s / L[a][a];
+#pragma unroll
+        for (int a = 0; a < NB; ++a) {
+            T p[4] = {x[a], T(0), T(0), T(0)};
+#pragma unroll
+            for (int b = 0; b < a; ++b) p[b & 3] -= L[a][b] * x[b];
+            x[a] = a < nb ? ((p[0] + p[1]) + (p[2] + p[3])) / L[a][a] : T(0);
         }
     } else {
-        for (int a = nb - 1; a >= 0; --a) {
-            T s = x[a];
-            for (int b = a + 1; b < nb; ++b) s -= L[b][a] * x[b];
-            x[a] = s / L[a][a];
+#pragma unroll
+        for (int a = NB - 1; a >= 0; --a) {
+            T p[4] = {x[a], T(0), T(0), T(0)};
+#pragma unroll
+            for (int b = a + 1; b < NB; ++b) p[b & 3] -= L[b][a] * x[b];
+            x[a] = a < nb ? ((p[0] + p[1]) + (p[2] + p[3])) / L[a][a] : T(0);
         }
     }
-    for (int a = 0; a < nb; ++a) B[(size_t)(i0 + a) * ldb + c] = x[a];
+#pragma unroll
+    for (int a = 0; a < NB; ++a)
+        if (a < nb) bc[(size_t)a * rs] = x[a];
 }
 
 template <typename T>
@@ -146,12 +129,13 @@ static int potrf_impl(int n, T* A, int lda, int* status, void* stream) {
     const int mode = sizeof(T) == 8 ? RVGP_GEMM_F64 : RVGP_GEMM_F32;
     for (int j0 = 0; j0 < n; j0 += NB) {
         const int nb = min(NB, n - j0);
-        hipLaunchKernelGGL(potrf_diag<T>, dim3(1), dim3(256), 0, s, A, lda, j0, nb, status);
+        hipLaunchKernelGGL(potrf_diag<T>, dim3(1), dim3(64), 0, s, A, lda, j0, nb, status);
         RVGP_LAUNCH_CHECK();
         const int rest = n - j0 - nb;
         if (rest <= 0) break;
-        hipLaunchKernelGGL(potrf_panel<T>, dim3(cdiv(rest, 64)), dim3(64), 0, s, A, lda, n, j0, nb,
-                           status);
+        // panel: rows [j0+nb, n) of block column j0 solve x L11^T = a
+        hipLaunchKernelGGL(trsm_diag<T>, dim3(cdiv(rest, 256)), dim3(256), 0, s, 0, A, lda, j0, nb,
+                           rest, A + (size_t)(j0 + nb) * lda + j0, 1LL, (long long)lda, status);
         RVGP_LAUNCH_CHECK();
         T* A21 = A + (size_t)(j0 + nb) * lda + j0;
         T* A22 = A + (size_t)(j0 + nb) * lda + j0 + nb;
@@ -172,22 +156,24 @@ static int trsm_impl(int trans, int n, int r, const T* L, int lda, T* B, int ldb
     for (int t = 0; t < nblk; ++t) {
         const int bi = trans ? nblk - 1 - t : t;
         const int i0 = bi * NB, nb = min(NB, n - i0);
-        if (!trans && i0 > 0) {
-            // B_i -= L[i, :i] X[:i]
-            int rc = rvgp_gemm(mode, 0, 0, nb, r, i0, -1.0, L + (size_t)i0 * lda, lda, B, ldb,
-                               nullptr, 1.0, B + (size_t)i0 * ldb, ldb, 1, nullptr, 0, stream);
-            if (rc) return rc;
-        } else if (trans && i0 + nb < n) {
-            // B_i -= L[i+nb:, i]^T X[i+nb:]
+        hipLaunchKernelGGL(trsm_diag<T>, dim3(cdiv(r, 256)), dim3(256), 0, s, trans, L, lda, i0, nb, r,
+                           B + (size_t)i0 * ldb, (long long)ldb, 1LL, (const int*)nullptr);
+        RVGP_LAUNCH_CHECK();
+        // Right-looking update with the block just solved: every update has
+        // K = NB and spans all remaining rows, so it fills many tiles.
+        if (!trans && i0 + nb < n) {
+            // B[i+nb:] -= L[i+nb:, i] X_i
             const int rest = n - i0 - nb;
-            int rc = rvgp_gemm(mode, 1, 0, nb, r, rest, -1.0, L + (size_t)(i0 + nb) * lda + i0, lda,
-                               B + (size_t)(i0 + nb) * ldb, ldb, nullptr, 1.0,
-                               B + (size_t)i0 * ldb, ldb, 1, nullptr, 0, stream);
+            int rc = rvgp_gemm(mode, 0, 0, rest, r, nb, -1.0, L + (size_t)(i0 + nb) * lda + i0, lda,
+                               B + (size_t)i0 * ldb, ldb, nullptr, 1.0,
+                               B + (size_t)(i0 + nb) * ldb, ldb, 1, nullptr, 0, stream);
+            if (rc) return rc;
+        } else if (trans && i0 > 0) {
+            // B[:i] -= L[i, :i]^T X_i
+            int rc = rvgp_gemm(mode, 1, 0, i0, r, nb, -1.0, L + (size_t)i0 * lda, lda,
+                               B + (size_t)i0 * ldb, ldb, nullptr, 1.0, B, ldb, 1, nullptr, 0, stream);
             if (rc) return rc;
         }
-        hipLaunchKernelGGL(trsm_diag<T>, dim3(cdiv(r, 64)), dim3(64), 0, s, trans, L, lda, i0, nb, r,
-                           B, ldb);
-        RVGP_LAUNCH_CHECK();
     }
     return RVGP_OK;
 }

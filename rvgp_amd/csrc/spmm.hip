@@ -10,15 +10,20 @@
 // CSR order (rowptr / cols over block indices).  Dense operands are
 // (nb*b, r) row-major with explicit row strides.
 //
-// Two mappings:
+// Three mappings:
 //  * narrow (r <= 4): LPR lanes share one block row and split its blocks
 //    (nnz-parallel), reducing with cross-lane shuffles -- consecutive lanes
 //    read consecutive blocks, so block values stream coalesced.
-//  * wide (r > 4): one wavefront per (block row, 64*CPL-column chunk); the
-//    block row index is wave-uniform, so block values and column indices
-//    are scalar loads, and every X row segment is a 64-lane coalesced load.
-//    Column chunks form the slow grid dimension so that all block rows of
-//    one chunk run together and the chunk of X they gather stays in cache.
+//  * vector (r > 4, 16-byte aligned rows): one wavefront per block row and
+//    up to 3 (f32) / 5 (f64) spans of 64 16-byte vectors; each X row segment
+//    is one 1 KiB coalesced load.
+//  * wide (r > 4, otherwise): one wavefront per (block row, 64*CPL-column
+//    chunk), one 4/8-byte column per lane per load.
+// In the wide forms the block row index is wave-uniform: the row's column
+// indices and block values are fetched coalesced and broadcast with
+// v_readlane.  Column chunks form the slow grid dimension so that all block
+// rows of one chunk run together and the chunk of X they gather stays in
+// cache.
 #include "common.h"
 
 namespace rvgp {
@@ -84,13 +89,33 @@ __global__ __launch_bounds__(256) void spmm_narrow(
         }
 }
 
-template <typename T, int B, int CPL>
-__global__ __launch_bounds__(256) void spmm_wide(
+// Wide SpMM, vector form (16-byte rows).  When r, every leading dimension and
+// every base pointer are multiples of the vector width VW (4 f32 / 2 f64),
+// lane l owns the VW consecutive columns VW*l + 64*VW*t of its chunk, so one
+// gather instruction moves 1 KiB of an X row instead of 256 B, and one wave
+// covers up to NCH*64*VW columns of its block row.  Block values and column
+// indices are loaded and broadcast as in spmm_wide2.
+template <typename T, int VW>
+struct VecOf;
+template <>
+struct VecOf<float, 4> {
+    typedef float type __attribute__((ext_vector_type(4)));
+};
+template <>
+struct VecOf<double, 2> {
+    typedef double type __attribute__((ext_vector_type(2)));
+};
+
+template <typename T, int B, int VW, int NCH>
+__global__ __launch_bounds__(256) void spmm_vec(
     int nb, const int* __restrict__ rowptr, const int* __restrict__ cols,
     const T* __restrict__ vals, const T* __restrict__ X, int ldx, int r, T alpha, T beta,
     const T* __restrict__ Z, int ldz, T gamma, T* __restrict__ Y, int ldy, T* __restrict__ acc,
     int ldacc, T delta) {
+    using V = typename VecOf<T, VW>::type;
     constexpr int WPB = 4;
+    constexpr int BB = B * B;
+    constexpr int SPAN = 64 * VW;
     const int lane = threadIdx.x & 63;
     // XCD-aware mapping: workgroups are dealt round-robin to the 8 XCDs, so
     // workgroup w runs on XCD w % 8; give each XCD one contiguous range of
@@ -100,33 +125,49 @@ __global__ __launch_bounds__(256) void spmm_wide(
     const int lw = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
     const int row = __builtin_amdgcn_readfirstlane(lw * WPB + (threadIdx.x >> 6));
     if (row >= nb) return;
-    const int c0 = blockIdx.y * (64 * CPL) + lane;
-    T y[B][CPL];
+    const int cb = blockIdx.y * (SPAN * NCH) + VW * lane;
+    V y[B][NCH];
 #pragma unroll
     for (int a = 0; a < B; ++a)
 #pragma unroll
-        for (int t = 0; t < CPL; ++t) y[a][t] = T(0);
+        for (int t = 0; t < NCH; ++t) y[a][t] = V(0);
     const int e0 = rowptr[row], e1 = rowptr[row + 1];
-    for (int e = e0; e < e1; ++e) {
-        const int j = cols[e];
-        const T* blk = vals + (size_t)e * B * B;
-        T v[B][B];
+    for (int base = e0; base < e1; base += 64) {
+        const int cnt = min(64, e1 - base);
+        const int mycol = lane < cnt ? cols[base + lane] : 0;
+        T mv[BB];
 #pragma unroll
-        for (int a = 0; a < B; ++a)
+        for (int q = 0; q < BB; ++q) {
+            const int f = lane + 64 * q;
+            mv[q] = f < cnt * BB ? vals[(size_t)base * BB + f] : T(0);
+        }
 #pragma unroll
-            for (int bb = 0; bb < B; ++bb) v[a][bb] = blk[a * B + bb];
-        const T* xr = X + (size_t)j * B * ldx;
-#pragma unroll
-        for (int t = 0; t < CPL; ++t) {
-            const int c = c0 + 64 * t;
-            if (c < r) {
-                T xb[B];
-#pragma unroll
-                for (int bb = 0; bb < B; ++bb) xb[bb] = xr[(size_t)bb * ldx + c];
+        for (int e = 0; e < 64; ++e) {
+            if (e < cnt) {
+                const int j = __builtin_amdgcn_readlane(mycol, e);
+                T v[B][B];
 #pragma unroll
                 for (int a = 0; a < B; ++a)
 #pragma unroll
-                    for (int bb = 0; bb < B; ++bb) y[a][t] += v[a][bb] * xb[bb];
+                    for (int bb = 0; bb < B; ++bb) {
+                        const int f = e * BB + a * B + bb;  // constant after unrolling
+                        v[a][bb] = bcast(mv[f >> 6], f & 63);
+                    }
+                const T* xr = X + (size_t)j * B * ldx;
+#pragma unroll
+                for (int t = 0; t < NCH; ++t) {
+                    const int c = cb + SPAN * t;
+                    if (c < r) {
+                        V xb[B];
+#pragma unroll
+                        for (int bb = 0; bb < B; ++bb)
+                            xb[bb] = *reinterpret_cast<const V*>(xr + (size_t)bb * ldx + c);
+#pragma unroll
+                        for (int a = 0; a < B; ++a)
+#pragma unroll
+                            for (int bb = 0; bb < B; ++bb) y[a][t] += v[a][bb] * xb[bb];
+                    }
+                }
             }
         }
     }
@@ -134,28 +175,21 @@ __global__ __launch_bounds__(256) void spmm_wide(
     for (int a = 0; a < B; ++a) {
         const size_t rr = (size_t)(row * B + a);
 #pragma unroll
-        for (int t = 0; t < CPL; ++t) {
-            const int c = c0 + 64 * t;
+        for (int t = 0; t < NCH; ++t) {
+            const int c = cb + SPAN * t;
             if (c < r) {
-                T out = alpha * y[a][t];
-                if (beta != T(0)) out += beta * X[rr * ldx + c];
-                if (Z != nullptr && gamma != T(0)) out += gamma * Z[rr * ldz + c];
-                Y[rr * ldy + c] = out;
-                if (acc != nullptr) acc[rr * ldacc + c] += delta * out;
+                V out = alpha * y[a][t];
+                if (beta != T(0)) out += beta * *reinterpret_cast<const V*>(X + rr * ldx + c);
+                if (Z != nullptr && gamma != T(0))
+                    out += gamma * *reinterpret_cast<const V*>(Z + rr * ldz + c);
+                *reinterpret_cast<V*>(Y + rr * ldy + c) = out;
+                if (acc != nullptr) {
+                    V* ap = reinterpret_cast<V*>(acc + rr * ldacc + c);
+                    *ap = *ap + delta * out;
+                }
             }
         }
     }
-}
-
-// Broadcast lane `l` of v (l wave-uniform) -- v_readlane into an SGPR.
-__device__ __forceinline__ float bcast(float v, int l) {
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
-}
-__device__ __forceinline__ double bcast(double v, int l) {
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), l);
-    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
-    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
 // Wide SpMM, latency-hiding form.  One wavefront per (block row, column
@@ -241,6 +275,17 @@ __global__ __launch_bounds__(256) void spmm_wide2(
     }
 }
 
+// The vector form needs r, every leading dimension and every base address to
+// be multiples of the 16-byte vector.
+template <typename T>
+static bool vec_ok(int r, const T* X, int ldx, const T* Y, int ldy, const T* Z, int ldz,
+                   const T* acc, int ldacc) {
+    constexpr int VW = 16 / sizeof(T);
+    auto al = [](const T* p, int ld) { return ((uintptr_t)p % 16) == 0 && ld % VW == 0; };
+    return r % VW == 0 && al(X, ldx) && al(Y, ldy) && (Z == nullptr || al(Z, ldz)) &&
+           (acc == nullptr || al(acc, ldacc));
+}
+
 template <typename T>
 static int spmm_dispatch(int nb, int b, const int* rowptr, const int* cols, const T* vals,
                          const T* X, int ldx, int r, T alpha, T beta, const T* Z, int ldz,
@@ -263,6 +308,29 @@ static int spmm_dispatch(int nb, int b, const int* rowptr, const int* cols, cons
         else if (b == 2) RVGP_NARROW(2);
         else RVGP_NARROW(3);
 #undef RVGP_NARROW
+    } else if (vec_ok<T>(r, X, ldx, Y, ldy, Z, ldz, acc, ldacc)) {
+        constexpr int VW = 16 / sizeof(T);
+        constexpr int NCHMAX = sizeof(T) == 4 ? 3 : 5;  // 64-lane spans per wave
+        const int spans = (int)cdiv(r / VW, 64);
+        const int groups = (spans + NCHMAX - 1) / NCHMAX;
+        const int nch = (spans + groups - 1) / groups;
+        dim3 g(8 * cdiv(cdiv(nb, 4), 8), groups);
+#define RVGP_VEC(BB, NN)                                                                           \
+    hipLaunchKernelGGL((spmm_vec<T, BB, VW, NN>), g, dim3(256), 0, s, nb, rowptr, cols, vals, X, \
+                       ldx, r, alpha, beta, Z, ldz, gamma, Y, ldy, acc, ldacc, delta)
+#define RVGP_VEC_B(NN)                 \
+    if (b == 1) RVGP_VEC(1, NN);       \
+    else if (b == 2) RVGP_VEC(2, NN);  \
+    else RVGP_VEC(3, NN);
+        if (nch == 1) { RVGP_VEC_B(1) }
+        else if (nch == 2) { RVGP_VEC_B(2) }
+        else if (nch == 3) { RVGP_VEC_B(3) }
+        else if constexpr (NCHMAX >= 5) {
+            if (nch == 4) { RVGP_VEC_B(4) }
+            else { RVGP_VEC_B(5) }
+        }
+#undef RVGP_VEC_B
+#undef RVGP_VEC
     } else {
         // columns per lane: balance the column chunks (at most 4 f32 / 2 f64)
         const int cmax = sizeof(T) == 4 ? 4 : 2;

@@ -48,12 +48,15 @@ def _rayleigh_ritz(A, Y, work_dtype):
     H = dense.gemm(Y, W, transA=True, out_dtype=torch.float64)
     G = 0.5 * (G + G.T)
     H = 0.5 * (H + H.T)
-    g, V = torch.linalg.eigh(G)
-    gmax = float(g[-1].item())
     drop = 1e-13 if work_dtype == torch.float64 else 1e-10
-    keep = g > gmax * drop
-    V = V[:, keep]
-    Bm = V / torch.sqrt(g[keep])
+    Bm = _gram_basis_chol(G, drop)
+    if Bm is None:
+        # numerically dependent block: eigendecomposition of G drops the
+        # directions below the rounding level instead of failing
+        g, V = torch.linalg.eigh(G)
+        gmax = float(g[-1].item())
+        keep = g > gmax * drop
+        Bm = V[:, keep] / torch.sqrt(g[keep])
     Hb = Bm.T @ H @ Bm
     Hb = 0.5 * (Hb + Hb.T)
     theta, Q = torch.linalg.eigh(Hb)
@@ -63,11 +66,26 @@ def _rayleigh_ritz(A, Y, work_dtype):
     return X, theta, AX
 
 
+def _gram_basis_chol(G, drop):
+    """B with B^T G B = I from the Cholesky factor G = L L^T (B = L^-T), on the
+    matrix cores; None when G is not numerically positive definite at the
+    level `drop` (smallest pivot^2 below drop * largest diagonal)."""
+    p = G.shape[0]
+    L = G.clone()
+    _, st = dense.potrf(L, check=False)
+    d = torch.diagonal(L)
+    ok = torch.logical_and(st[0] == 0, (d.min() ** 2) > drop * torch.diagonal(G).max())
+    if not bool(ok.item()):
+        return None
+    Linv = torch.eye(p, dtype=G.dtype, device=G.device)
+    dense.trsm(torch.tril(L).contiguous(), Linv)
+    return Linv.T.contiguous()
+
+
 def _residuals(X, theta, AX):
+    """|A x_j - theta_j x_j| / |x_j| per column (column sums in FP64)."""
     R = AX - X * theta.to(X.dtype)
-    rn = torch.linalg.vector_norm(R.double(), dim=0)
-    xn = torch.linalg.vector_norm(X.double(), dim=0)
-    return rn / xn
+    return torch.sqrt(dense.colsumsq(R) / dense.colsumsq(X))
 
 
 def _cheb_filter(A, X, m, a, b, a0):
@@ -156,6 +174,7 @@ def _run_phase(A, X, k, ub, wdt, ptol, max_iter, max_ratio, lock, fresh, stats, 
             while c < kk and conv[c]:
                 c += 1
             c = min(c, max(0, Xa.shape[1] - 8))  # keep a working block
+            c -= c % 8  # block widths stay multiples of 8: 16-byte SpMM rows
             if c > 0:
                 Xl = torch.cat([Xl, Xa[:, :c]], 1).contiguous()
                 thl = torch.cat([thl, tha[:c]])
@@ -188,7 +207,7 @@ def chfsi(A: BSR, k, precision="fp64", tol=1e-9, max_iter=300, guard=None, seed=
     dev = A.rowptr.device
     if guard is None:
         guard = max(8, int(math.ceil(0.15 * k)))
-    p = min(N, k + guard)
+    p = min(N, -(-(k + guard) // 8) * 8)
     ub = A.gershgorin_bound() * 1.0001 + 1e-12
     stats = EigStats(spmm=0, rr=0, iters=0, t_filter=0.0, t_rr=0.0)
     gen = torch.Generator(device=dev)
@@ -228,7 +247,7 @@ def chfsi(A: BSR, k, precision="fp64", tol=1e-9, max_iter=300, guard=None, seed=
     stats["residual"] = float(_residuals(X[:, :k], theta[:k], AX).max().item() / ub)
     evals = theta[:k].clone()
     evecs = X[:, :k].double()
-    evecs = evecs / torch.linalg.vector_norm(evecs, dim=0)
+    evecs = evecs / torch.sqrt(dense.colsumsq(evecs))
     return evals, evecs, stats
 
 

@@ -124,12 +124,13 @@ class GPREngine:
             _lib.call("rvgp_ws_form", self.k, _lib.ptr(self.G), self.k, _lib.ptr(self.S),
                       _lib.ptr(self.s2), _lib.ptr(self.A), self.k, s)
             _lib.call("rvgp_potrf", 1, self.k, _lib.ptr(self.A), self.k, _lib.ptr(self.status), s)
-            self.w.copy_(self.b)
-            dense.trsm(self.A, self.w)
-            dense.trsm(self.A, self.w, trans=True)
+            # L^-1 once (its column norms are diag(A^-1)); w = L^-T (L^-1 b) by two
+            # GEMVs instead of two more blocked triangular solves
             self.Linv.copy_(self.eye)
             dense.trsm(self.A, self.Linv)
             self._colsumsq(self.Linv, self.ainv)
+            t = dense.gemm(self.Linv, self.b)
+            dense.gemm(self.Linv, t, self.w, transA=True)
         if self.form == "weight":
             Gw = dense.gemm(self.G, self.w)
             _lib.call("rvgp_ws_finish", self.k, float(self.M), _lib.ptr(self.A), self.k,

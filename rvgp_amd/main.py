@@ -85,7 +85,8 @@ class manifold_GPR:
 def train_gp(data, train_ind=None, n_inducing_points=None, test_size=0.2, kernel=None,
              noise_variance=0.001, kernel_lengthscale=None, kernel_variance=None, epochs=1000,
              positivity_constraint=1e-2, seed=0, *, reference_semantics=True, optimizer="lbfgs",
-             lr=0.05, form="auto", precision="fp64", verbose=True, use_graph=True):
+             lr=0.05, form="auto", precision="fp64", verbose=True, use_graph=True,
+             timings=None):
     """Fit the vector-field GP (main.py:11-84)."""
     if train_ind is None:
         train_ind = np.arange(data.n)
@@ -115,6 +116,7 @@ def train_gp(data, train_ind=None, n_inducing_points=None, test_size=0.2, kernel
         raise AttributeError("'ManifoldKernel' object has no attribute 'lengthscales'")
 
     dev = _lib.device()
+    clock = _Clock(timings)
     nd_tr = torch.as_tensor(nodes_tr.astype(np.int32), device=dev)
     Phi = data.phi_rows(nd_tr)
     V = data.vectors_device()
@@ -127,6 +129,7 @@ def train_gp(data, train_ind=None, n_inducing_points=None, test_size=0.2, kernel
     eng = GPREngine(Phi, y, data.spectrum_device(), kernel.num_verticies, kernel.typ, form=form,
                     precision=precision, lower=lower)
     eng.set_theta([float(kernel.nu), float(kernel.kappa), float(kernel.sigma_f), noise0])
+    clock.tick("gp_setup")
     if optimizer == "lbfgs":
         eng.fit_lbfgs(maxiter=epochs, disp=verbose)
     elif optimizer == "adam":
@@ -141,6 +144,7 @@ def train_gp(data, train_ind=None, n_inducing_points=None, test_size=0.2, kernel
         torch.cuda.synchronize()
     else:
         raise ValueError("optimizer must be 'lbfgs' or 'adam'")
+    clock.tick("gp_optimise")
     th = eng.theta_host()
     kernel.nu.assign(th[0])
     kernel.kappa.assign(th[1])
@@ -155,4 +159,22 @@ def train_gp(data, train_ind=None, n_inducing_points=None, test_size=0.2, kernel
         model.heldout_error = err
         if verbose:
             print("Relative l2 error is {}".format(err))
+    clock.tick("gp_heldout_predict")
     return model
+
+
+class _Clock:
+    """Optional stage timer (synchronises the device at each tick)."""
+
+    def __init__(self, timings):
+        import time
+        self.t, self.time = timings, time
+        self.last = time.perf_counter()
+
+    def tick(self, name):
+        if self.t is None:
+            return
+        torch.cuda.synchronize()
+        now = self.time.perf_counter()
+        self.t[name] = now - self.last
+        self.last = now

@@ -76,10 +76,23 @@ def spmm(A, X, Y=None, alpha=1.0, beta=0.0, Z=None, gamma=0.0, acc=None, delta=0
     if ev is not None:
         s = X2.element_size()
         nbytes = spmm_bytes(A, r, s, Z is not None and gamma != 0.0, acc is not None)
-        kind = "spmm_narrow" if r <= 4 else "spmm_wide"
+        kind = "spmm_narrow" if r <= 4 else (
+            "spmm_vec" if _vec_ok(r, X2, Y2, Z2, A2) else "spmm_wide")
         instrument.end(ev, kind, (str(X2.dtype).replace("torch.", ""), A.b), nbytes,
                        2.0 * A.nnzb * A.b * A.b * r)
     return Y
+
+
+def _vec_ok(r, *ts):
+    """Mirror of the kernel's test for the 16-byte vector form: r, every row
+    stride and every base address multiples of 16 bytes."""
+    for t in ts:
+        if t is None:
+            continue
+        vw = 16 // t.element_size()
+        if r % vw or t.stride(0) % vw or t.data_ptr() % 16:
+            return False
+    return True
 
 
 def spmm_bytes(A, r, s, with_z=False, with_acc=False):

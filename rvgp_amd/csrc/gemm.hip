@@ -196,7 +196,7 @@ __global__ __launch_bounds__(GT) void gemm_kernel(int transA, int transB, int M,
 constexpr int BBM = 128, BBN = 128;
 
 template <Mode MD>
-__global__ __launch_bounds__(GT) void gemm_big_kernel(int transA, int transB, int M, int N, int K,
+__global__ __launch_bounds__(GT) __attribute__((amdgpu_waves_per_eu(2))) void gemm_big_kernel(int transA, int transB, int M, int N, int K,
                                                       int kchunk, typename Tr<MD>::Acc alpha,
                                                       const typename Tr<MD>::In* __restrict__ A,
                                                       int lda,
@@ -209,7 +209,10 @@ __global__ __launch_bounds__(GT) void gemm_big_kernel(int transA, int transB, in
     using Lds = typename Tr<MD>::Lds;
     using Acc = typename Tr<MD>::Acc;
     using Vec = typename Tr<MD>::Vec;
-    using Reg = typename std::conditional<MD == Mode::BF16, float, Lds>::type;
+    // The prefetch registers hold the raw input: widening (f32 -> f64), the
+    // k-scale and the bf16 rounding happen at the LDS store, after the loads
+    // have landed, so the wait for them stays one k-slice behind their issue.
+    using Reg = In;
     constexpr int BK = Tr<MD>::BK;
     constexpr int PAD = (sizeof(Lds) == 8) ? 1 : 2;
     constexpr int EA = BBM * BK / GT;  // elements of each tile staged per thread
@@ -238,12 +241,8 @@ __global__ __launch_bounds__(GT) void gemm_big_kernel(int transA, int transB, in
             if (tA) { kk = e / BBM; mm = e - kk * BBM; }
             else { mm = e / BK; kk = e - mm * BK; }
             const int gm = m0 + mm, gk = k0 + kk;
-            double v = 0.0;
-            if (gm < M && gk < kend) {
-                v = (double)(tA ? A[(size_t)gk * lda + gm] : A[(size_t)gm * lda + gk]);
-                if (kscale) v *= kscale[gk];
-            }
-            ra[q] = (Reg)v;
+            ra[q] = (gm < M && gk < kend) ? (tA ? A[(size_t)gk * lda + gm] : A[(size_t)gm * lda + gk])
+                                          : In(0);
             int nn;
             if (tB) { nn = e / BK; kk = e - nn * BK; }
             else { kk = e / BBN; nn = e - kk * BBN; }
@@ -253,14 +252,20 @@ __global__ __launch_bounds__(GT) void gemm_big_kernel(int transA, int transB, in
             rb[q] = (Reg)u;
         }
     };
-    auto store = [&]() {
+    auto store = [&](int k0) {
 #pragma unroll
         for (int q = 0; q < EA; ++q) {
             const int e = tid + GT * q;
             int mm, kk;
             if (tA) { kk = e / BBM; mm = e - kk * BBM; }
             else { mm = e / BK; kk = e - mm * BK; }
-            As[kk][mm] = static_cast<Lds>(ra[q]);
+            if (kscale != nullptr && k0 + kk < kend) {
+                const double v = (double)ra[q] * kscale[k0 + kk];
+                if constexpr (MD == Mode::F32 || MD == Mode::BF16) As[kk][mm] = to_lds<MD>((float)v);
+                else As[kk][mm] = (Lds)v;
+            } else {
+                As[kk][mm] = to_lds<MD>(ra[q]);
+            }
             int nn;
             if (tB) { nn = e / BK; kk = e - nn * BK; }
             else { kk = e / BBN; nn = e - kk * BBN; }
@@ -270,7 +275,7 @@ __global__ __launch_bounds__(GT) void gemm_big_kernel(int transA, int transB, in
 
     if (kbeg < kend) load(kbeg);
     for (int k0 = kbeg; k0 < kend; k0 += BK) {
-        store();
+        store(k0);
         __syncthreads();
         if (k0 + BK < kend) load(k0 + BK);
         if constexpr (MD == Mode::BF16) {
@@ -351,14 +356,149 @@ __global__ void splitk_reduce(int M, int N, int S, Acc alpha, const Acc* __restr
 // ~1024 workgroups (the tall-skinny Grams have few output tiles), at least 8
 // k-steps per slice, at most 256 slices; the split depends only on the shape,
 // so results are reproducible.
+// Small GEMM for the latency-bound regime (few output tiles, K <= 1024: the
+// Cholesky / triangular-solve updates and matrix-vector products of the GP
+// step).  One wavefront per 16 x 16 output tile; the MFMA operands are loaded
+// straight from global memory into registers (no LDS, no barriers), 16
+// k-steps per batch with the next batch's loads in flight during the current
+// batch's MFMAs, and C (when beta != 0) fetched before the k loop.  `lower`
+// restricts the output to the lower triangle (Cholesky trailing update).
+constexpr int SU = 16;
+
+template <Mode MD>
+__global__ __launch_bounds__(256) void gemm_small_kernel(
+    int transA, int transB, int M, int N, int K, typename Tr<MD>::Acc alpha,
+    const typename Tr<MD>::In* __restrict__ A, int lda, const typename Tr<MD>::In* __restrict__ B,
+    int ldb, const double* __restrict__ kscale, typename Tr<MD>::Acc beta,
+    typename Tr<MD>::Acc* __restrict__ C, int ldc, int lower, int tiles_n) {
+    using In = typename Tr<MD>::In;
+    using Acc = typename Tr<MD>::Acc;
+    using Vec = typename Tr<MD>::Vec;
+    using Op = typename Tr<MD>::Lds;  // MFMA operand type: f64, or f32 for Mode::F32
+    constexpr bool F64OUT = sizeof(Op) == 8;
+    const int lane = threadIdx.x & 63;
+    const int tile = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int ti = tile / tiles_n, tj = tile - ti * tiles_n;
+    const int m0 = ti * 16, n0 = tj * 16;
+    if (m0 >= M) return;
+    if (lower && n0 > m0 + 15) return;  // tile strictly above the diagonal
+    const int am = m0 + (lane & 15), bn = n0 + (lane & 15), kq = lane >> 4;
+    const bool mok = am < M, nok = bn < N;
+
+    Acc cpre[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int row = m0 + (F64OUT ? (lane >> 4) + 4 * r : 4 * (lane >> 4) + r);
+        cpre[r] = (beta != Acc(0) && row < M && nok) ? C[(size_t)row * ldc + bn] : Acc(0);
+    }
+    In ra0[SU], rb0[SU], ra1[SU], rb1[SU];
+    auto load = [&](In (&ra)[SU], In (&rb)[SU], int k0) {
+#pragma unroll
+        for (int u = 0; u < SU; ++u) {
+            const int k = k0 + 4 * u + kq;
+            ra[u] = (mok && k < K) ? (transA ? A[(size_t)k * lda + am] : A[(size_t)am * lda + k]) : In(0);
+            rb[u] = (nok && k < K) ? (transB ? B[(size_t)bn * ldb + k] : B[(size_t)k * ldb + bn]) : In(0);
+        }
+    };
+    Vec acc = Vec{0, 0, 0, 0};
+    auto compute = [&](In (&ra)[SU], In (&rb)[SU], int k0) {
+#pragma unroll
+        for (int u = 0; u < SU; ++u) {
+            if (k0 + 4 * u < K) {  // wave-uniform
+                Op a = (Op)ra[u];
+                const int k = k0 + 4 * u + kq;
+                if (kscale != nullptr && k < K) a = (Op)((double)ra[u] * kscale[k]);
+                if constexpr (F64OUT)
+                    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, (Op)rb[u], acc, 0, 0, 0);
+                else
+                    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, (Op)rb[u], acc, 0, 0, 0);
+            }
+        }
+    };
+    load(ra0, rb0, 0);
+    for (int k0 = 0; k0 < K; k0 += 8 * SU) {
+        if (k0 + 4 * SU < K) load(ra1, rb1, k0 + 4 * SU);
+        compute(ra0, rb0, k0);
+        if (k0 + 4 * SU >= K) break;
+        if (k0 + 8 * SU < K) load(ra0, rb0, k0 + 8 * SU);
+        compute(ra1, rb1, k0 + 4 * SU);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int row = m0 + (F64OUT ? (lane >> 4) + 4 * r : 4 * (lane >> 4) + r);
+        if (row < M && nok && (!lower || bn <= row)) {
+            Acc o = alpha * acc[r];
+            if (beta != Acc(0)) o += beta * cpre[r];
+            C[(size_t)row * ldc + bn] = o;
+        }
+    }
+}
+
+// Matrix-vector product (N = 1, K <= 8192, no k-scale):
+//     y = alpha op(A) x + beta y.
+// op = N: one wavefront per row, lanes striding K (coalesced), wave sum.
+// op = T: a workgroup per 64 outputs, lane = output column (coalesced along
+// the row), the four waves splitting K and reducing through LDS in a fixed
+// order.  Accumulates in Acc (f64, or f32 for Mode::F32).
+template <Mode MD>
+__global__ __launch_bounds__(256) void gemv_kernel(int transA, int M, int K,
+                                                   typename Tr<MD>::Acc alpha,
+                                                   const typename Tr<MD>::In* __restrict__ A,
+                                                   int lda, const typename Tr<MD>::In* __restrict__ x,
+                                                   int incx, typename Tr<MD>::Acc beta,
+                                                   typename Tr<MD>::Acc* __restrict__ y, int incy) {
+    using Acc = typename Tr<MD>::Acc;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (!transA) {
+        const int m = blockIdx.x * 4 + w;
+        if (m >= M) return;
+        const auto* row = A + (size_t)m * lda;
+        Acc p[4] = {Acc(0), Acc(0), Acc(0), Acc(0)};
+        int k = lane, t = 0;
+        for (; k < K; k += 64, t = (t + 1) & 3) p[t] += (Acc)row[k] * (Acc)x[(size_t)k * incx];
+        Acc s = wave_sum((p[0] + p[1]) + (p[2] + p[3]));
+        if (lane == 0) {
+            Acc o = alpha * s;
+            if (beta != Acc(0)) o += beta * y[(size_t)m * incy];
+            y[(size_t)m * incy] = o;
+        }
+    } else {
+        __shared__ Acc part[4][64];
+        const int m = blockIdx.x * 64 + lane;
+        const int per = (K + 3) / 4, k0 = w * per, k1 = min(K, k0 + per);
+        Acc p[4] = {Acc(0), Acc(0), Acc(0), Acc(0)};
+        if (m < M) {
+            int t = 0;
+            for (int k = k0; k < k1; ++k, t = (t + 1) & 3)
+                p[t] += (Acc)A[(size_t)k * lda + m] * (Acc)x[(size_t)k * incx];
+        }
+        part[w][lane] = (p[0] + p[1]) + (p[2] + p[3]);
+        __syncthreads();
+        if (w == 0 && m < M) {
+            Acc o = alpha * ((part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]));
+            if (beta != Acc(0)) o += beta * y[(size_t)m * incy];
+            y[(size_t)m * incy] = o;
+        }
+    }
+}
+
 struct GemmPlan {
-    bool big;
+    bool small, big;
     int split, kchunk;
 };
 
-static GemmPlan gemm_plan(int M, int N, int K, int BK, int split_k) {
+// small: K <= 1024 and at most 64 tiles of the 128 x 128 kernel -- too few
+// workgroups to hide the k-loop latency, so one wave per 16 x 16 tile wins.
+static GemmPlan gemm_plan(int mode, int M, int N, int K, int BK, int split_k) {
     GemmPlan p;
+    p.small = mode != RVGP_GEMM_BF16 && split_k <= 1 && K <= 1024 &&
+              (long)cdiv(M, BBM) * cdiv(N, BBN) <= 64;
     p.big = M >= 96 && N >= 96;
+    if (p.small) {
+        p.split = 1;
+        p.kchunk = K;
+        return p;
+    }
     const long tiles = p.big ? (long)cdiv(M, BBM) * cdiv(N, BBN) : (long)cdiv(M, BM) * cdiv(N, BN);
     int split = split_k;
     if (split <= 0) {
@@ -382,7 +522,8 @@ static GemmPlan gemm_plan(int M, int N, int K, int BK, int split_k) {
 template <Mode MD>
 static int gemm_launch(int transA, int transB, int M, int N, int K, double alpha, const void* A,
                        int lda, const void* B, int ldb, const double* kscale, double beta,
-                       void* C, int ldc, int split, void* ws, size_t ws_bytes, void* stream) {
+                       void* C, int ldc, int split, void* ws, size_t ws_bytes, void* stream,
+                       int lower = 0) {
     using In = typename Tr<MD>::In;
     using Acc = typename Tr<MD>::Acc;
     RVGP_CHECK_ARG(M >= 0 && N >= 0 && K >= 0, "rvgp_gemm: negative size");
@@ -390,7 +531,39 @@ static int gemm_launch(int transA, int transB, int M, int N, int K, double alpha
     RVGP_CHECK_ARG(lda >= (transA ? M : K) && ldb >= (transB ? K : N), "rvgp_gemm: bad lda/ldb");
     if (M == 0 || N == 0) return RVGP_OK;
     hipStream_t s = as_stream(stream);
-    const GemmPlan pl = gemm_plan(M, N, K, Tr<MD>::BK, split);
+    constexpr int MODE = MD == Mode::F64 ? RVGP_GEMM_F64
+                         : MD == Mode::F32 ? RVGP_GEMM_F32
+                         : MD == Mode::F32ACC64 ? RVGP_GEMM_F32_ACC64 : RVGP_GEMM_BF16;
+    const GemmPlan pl = gemm_plan(MODE, M, N, K, Tr<MD>::BK, split);
+    if constexpr (MD != Mode::BF16) {
+        if (N == 1 && K <= 8192 && kscale == nullptr && !lower && split <= 1) {
+            const int incx = transB ? 1 : ldb;
+            if (transA)
+                hipLaunchKernelGGL(gemv_kernel<MD>, dim3(cdiv(M, 64)), dim3(256), 0, s, 1, M, K,
+                                   (Acc)alpha, static_cast<const In*>(A), lda,
+                                   static_cast<const In*>(B), incx, (Acc)beta,
+                                   static_cast<Acc*>(C), ldc);
+            else
+                hipLaunchKernelGGL(gemv_kernel<MD>, dim3(cdiv(M, 4)), dim3(256), 0, s, 0, M, K,
+                                   (Acc)alpha, static_cast<const In*>(A), lda,
+                                   static_cast<const In*>(B), incx, (Acc)beta,
+                                   static_cast<Acc*>(C), ldc);
+            RVGP_LAUNCH_CHECK();
+            return RVGP_OK;
+        }
+    }
+    if (pl.small) {
+        if constexpr (MD != Mode::BF16) {
+            const int tn = (int)cdiv(N, 16);
+            const size_t tiles = (size_t)cdiv(M, 16) * tn;
+            hipLaunchKernelGGL(gemm_small_kernel<MD>, dim3(cdiv(tiles, 4)), dim3(256), 0, s, transA,
+                               transB, M, N, K, (Acc)alpha, static_cast<const In*>(A), lda,
+                               static_cast<const In*>(B), ldb, kscale, (Acc)beta,
+                               static_cast<Acc*>(C), ldc, lower, tn);
+            RVGP_LAUNCH_CHECK();
+            return RVGP_OK;
+        }
+    }
     Acc* part = nullptr;
     if (pl.split > 1) {
         const size_t need = sizeof(Acc) * (size_t)pl.split * M * N;
@@ -450,6 +623,19 @@ extern "C" size_t rvgp_gemm_ws_bytes(int mode, int M, int N, int K, int split_k)
     if (M <= 0 || N <= 0) return 0;
     const int BK = mode == RVGP_GEMM_BF16 ? Tr<Mode::BF16>::BK : Tr<Mode::F64>::BK;
     const size_t acc = (mode == RVGP_GEMM_F64 || mode == RVGP_GEMM_F32_ACC64) ? 8 : 4;
-    const GemmPlan p = gemm_plan(M, N, K, BK, split_k);
+    const GemmPlan p = gemm_plan(mode, M, N, K, BK, split_k);
     return p.split > 1 ? acc * (size_t)p.split * M * N : 0;
 }
+
+namespace rvgp {
+// C = alpha A A^T + beta C on (at least) the lower triangle of the n x n C;
+// A is n x k row-major.  Internal: the Cholesky trailing update.
+int gemm_syrk_lower(int f64, int n, int k, double alpha, const void* A, int lda, double beta,
+                    void* C, int ldc, void* stream) {
+    if (f64)
+        return gemm_launch<Mode::F64>(0, 1, n, n, k, alpha, A, lda, A, lda, nullptr, beta, C, ldc,
+                                      1, nullptr, 0, stream, 1);
+    return gemm_launch<Mode::F32>(0, 1, n, n, k, alpha, A, lda, A, lda, nullptr, beta, C, ldc, 1,
+                                  nullptr, 0, stream, 1);
+}
+}  // namespace rvgp

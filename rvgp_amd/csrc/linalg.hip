@@ -4,10 +4,10 @@
 //
 // potrf: right-looking, NB = 64.  Per block column: (1) the diagonal block
 // is factorised by one wavefront with its rows in registers, (2) the panel
-// below is solved against it (one row per thread, L11 in LDS), (3) the
-// trailing matrix gets A22 -= A21 A21^T on the matrix cores (rvgp_gemm).
-// trsm: blocked substitution; the off-diagonal updates are GEMMs, the
-// diagonal-block solves one lane per right-hand-side column.
+// below is solved against it (one wavefront per panel row), (3) the lower
+// triangle of the trailing matrix gets A22 -= A21 A21^T on the matrix cores.
+// trsm: blocked right-looking substitution; the off-diagonal updates are
+// GEMMs, the diagonal-block solves one wavefront per right-hand side.
 #include "common.h"
 
 extern "C" int rvgp_gemm(int mode, int transA, int transB, int M, int N, int K, double alpha,
@@ -17,12 +17,21 @@ extern "C" int rvgp_gemm(int mode, int transA, int transB, int M, int N, int K, 
 
 namespace rvgp {
 
+int gemm_syrk_lower(int f64, int n, int k, double alpha, const void* A, int lda, double beta,
+                    void* C, int ldc, void* stream);
+
 constexpr int NB = 64;
 
 // Diagonal block, one wavefront: lane r holds row r of the block in
 // registers; the column loop is fully unrolled, so every register index is a
-// constant and the entries other lanes need (pivot, scaled column) arrive by
-// v_readlane.  No LDS and no barriers.  Rows past nb are identity rows.
+// constant and the entries other lanes need arrive by v_readlane.  The chain
+// that bounds the kernel is pivot -> 1/sqrt -> scaled column -> next pivot,
+// so column c+1 is updated first and its pivot's reciprocal square root is
+// in flight while the rest of the rank-1 update issues.  Entries right of a
+// lane's diagonal are scratch: a lane clears its entry c when column c is
+// scaled, so nothing it accumulates there reaches the factor, and the update
+// runs on every lane without a per-entry predicate.  Rows past nb are
+// identity rows.
 template <typename T>
 __global__ __launch_bounds__(64) void potrf_diag(T* __restrict__ A, int lda, int j0, int nb,
                                                  int* __restrict__ status) {
@@ -31,18 +40,18 @@ __global__ __launch_bounds__(64) void potrf_diag(T* __restrict__ A, int lda, int
 #pragma unroll
     for (int c = 0; c < NB; ++c)
         a[c] = (r < nb && c <= r) ? A[(size_t)(j0 + r) * lda + j0 + c] : (c == r ? T(1) : T(0));
+    T piv = bcast(a[0], 0);
     bool bad = false;
 #pragma unroll
     for (int c = 0; c < NB; ++c) {
-        const T piv = bcast(a[c], c);
         bad |= !(piv > T(0));
-        const T lcc = sqrt(piv);
-        if (r == c) a[c] = lcc;
-        if (r > c) a[c] = a[c] / lcc;
+        const T inv = rsqrt(piv);
+        a[c] = r == c ? piv * inv : (r > c ? a[c] * inv : T(0));
+        if (c + 1 < NB) {
+            a[c + 1] -= a[c] * bcast(a[c], c + 1);
+            piv = bcast(a[c + 1], c + 1);
 #pragma unroll
-        for (int s = c + 1; s < NB; ++s) {
-            const T lsc = bcast(a[c], s);
-            if (s <= r) a[s] -= a[c] * lsc;
+            for (int s = c + 2; s < NB; ++s) a[s] -= a[c] * bcast(a[c], s);
         }
     }
     if (bad) {  // wave-uniform: every lane saw the same pivots
@@ -60,10 +69,12 @@ __global__ __launch_bounds__(64) void potrf_diag(T* __restrict__ A, int lda, int
 // r right-hand sides: X = L11^-1 B (trans = 0) or L11^-T B (trans = 1), where
 // element (a, c) of B is B[a * rs + c * cs] -- rs = ldb, cs = 1 for the rows
 // [i0, i0+nb) of a row-major B, and rs = 1, cs = lda for the Cholesky panel,
-// whose rows are right-hand sides (x L11^T = a  <=>  L11 x^T = a^T).  One
-// right-hand side per thread, held in registers; L11 staged in LDS and read
-// as broadcasts.  Each dot product runs as four interleaved partial sums so
-// that the dependent FMA chain is a quarter as long.
+// whose rows are right-hand sides (x L11^T = a  <=>  L11 x^T = a^T).
+// One wavefront per right-hand side, lane a holding x_a: step a broadcasts
+// x_a = b_a / L_aa (v_readlane) and the lanes below (above, transposed)
+// subtract L_ja x_a -- a 64-step chain of readlane, multiply and FMA instead
+// of a serial 2016-FMA substitution per lane.  L11 and 1 / diag(L11) are
+// staged in LDS; 4 right-hand sides per workgroup.
 template <typename T>
 __global__ __launch_bounds__(256) void trsm_diag(int trans, const T* __restrict__ Lm, int lda,
                                                  int i0, int nb, int r, T* __restrict__ B,
@@ -71,41 +82,33 @@ __global__ __launch_bounds__(256) void trsm_diag(int trans, const T* __restrict_
                                                  const int* __restrict__ status) {
     if (status != nullptr && *status) return;
     __shared__ T L[NB][NB + 1];
+    __shared__ T dinv[NB];
     const int tid = threadIdx.x;
     for (int e = tid; e < NB * NB; e += 256) {  // whole tile: zeros outside the block
         int a = e / NB, b = e - a * NB;
         L[a][b] = (a < nb && b <= a) ? Lm[(size_t)(i0 + a) * lda + i0 + b] : T(0);
     }
+    if (tid < NB) dinv[tid] = tid < nb ? T(1) / Lm[(size_t)(i0 + tid) * lda + i0 + tid] : T(1);
     __syncthreads();
-    const int c = blockIdx.x * 256 + tid;
+    const int lane = tid & 63;
+    const int c = blockIdx.x * 4 + (tid >> 6);
     if (c >= r) return;
-    // The right-hand side lives in registers: every loop is unrolled over the
-    // fixed block size, so x[] is only ever indexed by constants.  Rows beyond
-    // nb carry x = 0, which leaves the solve unchanged.
     T* bc = B + (size_t)c * cs;
-    T x[NB];
-#pragma unroll
-    for (int a = 0; a < NB; ++a) x[a] = a < nb ? bc[(size_t)a * rs] : T(0);
+    T x = lane < nb ? bc[(size_t)lane * rs] : T(0);
     if (!trans) {
-#pragma unroll
-        for (int a = 0; a < NB; ++a) {
-            T p[4] = {x[a], T(0), T(0), T(0)};
-#pragma unroll
-            for (int b = 0; b < a; ++b) p[b & 3] -= L[a][b] * x[b];
-            x[a] = a < nb ? ((p[0] + p[1]) + (p[2] + p[3])) / L[a][a] : T(0);
+        for (int a = 0; a < nb; ++a) {
+            const T xa = bcast(x, a) * dinv[a];
+            if (lane == a) x = xa;
+            else if (lane > a) x -= L[lane][a] * xa;
         }
     } else {
-#pragma unroll
-        for (int a = NB - 1; a >= 0; --a) {
-            T p[4] = {x[a], T(0), T(0), T(0)};
-#pragma unroll
-            for (int b = a + 1; b < NB; ++b) p[b & 3] -= L[b][a] * x[b];
-            x[a] = a < nb ? ((p[0] + p[1]) + (p[2] + p[3])) / L[a][a] : T(0);
+        for (int a = nb - 1; a >= 0; --a) {
+            const T xa = bcast(x, a) * dinv[a];
+            if (lane == a) x = xa;
+            else if (lane < a) x -= L[a][lane] * xa;
         }
     }
-#pragma unroll
-    for (int a = 0; a < NB; ++a)
-        if (a < nb) bc[(size_t)a * rs] = x[a];
+    if (lane < nb) bc[(size_t)lane * rs] = x;
 }
 
 template <typename T>
@@ -126,7 +129,6 @@ template <typename T>
 static int potrf_impl(int n, T* A, int lda, int* status, void* stream) {
     RVGP_CHECK_ARG(n >= 0 && lda >= n, "rvgp_potrf: bad n/lda");
     hipStream_t s = as_stream(stream);
-    const int mode = sizeof(T) == 8 ? RVGP_GEMM_F64 : RVGP_GEMM_F32;
     for (int j0 = 0; j0 < n; j0 += NB) {
         const int nb = min(NB, n - j0);
         hipLaunchKernelGGL(potrf_diag<T>, dim3(1), dim3(64), 0, s, A, lda, j0, nb, status);
@@ -134,13 +136,12 @@ static int potrf_impl(int n, T* A, int lda, int* status, void* stream) {
         const int rest = n - j0 - nb;
         if (rest <= 0) break;
         // panel: rows [j0+nb, n) of block column j0 solve x L11^T = a
-        hipLaunchKernelGGL(trsm_diag<T>, dim3(cdiv(rest, 256)), dim3(256), 0, s, 0, A, lda, j0, nb,
+        hipLaunchKernelGGL(trsm_diag<T>, dim3(cdiv(rest, 4)), dim3(256), 0, s, 0, A, lda, j0, nb,
                            rest, A + (size_t)(j0 + nb) * lda + j0, 1LL, (long long)lda, status);
         RVGP_LAUNCH_CHECK();
         T* A21 = A + (size_t)(j0 + nb) * lda + j0;
         T* A22 = A + (size_t)(j0 + nb) * lda + j0 + nb;
-        int rc = rvgp_gemm(mode, 0, 1, rest, rest, nb, -1.0, A21, lda, A21, lda, nullptr, 1.0, A22,
-                           lda, 1, nullptr, 0, stream);
+        int rc = gemm_syrk_lower(sizeof(T) == 8, rest, nb, -1.0, A21, lda, 1.0, A22, lda, stream);
         if (rc) return rc;
     }
     return RVGP_OK;
@@ -156,7 +157,7 @@ static int trsm_impl(int trans, int n, int r, const T* L, int lda, T* B, int ldb
     for (int t = 0; t < nblk; ++t) {
         const int bi = trans ? nblk - 1 - t : t;
         const int i0 = bi * NB, nb = min(NB, n - i0);
-        hipLaunchKernelGGL(trsm_diag<T>, dim3(cdiv(r, 256)), dim3(256), 0, s, trans, L, lda, i0, nb, r,
+        hipLaunchKernelGGL(trsm_diag<T>, dim3(cdiv(r, 4)), dim3(256), 0, s, trans, L, lda, i0, nb, r,
                            B + (size_t)i0 * ldb, (long long)ldb, 1LL, (const int*)nullptr);
         RVGP_LAUNCH_CHECK();
         // Right-looking update with the block just solved: every update has

@@ -310,9 +310,12 @@ static int spmm_dispatch(int nb, int b, const int* rowptr, const int* cols, cons
 #undef RVGP_NARROW
     } else if (vec_ok<T>(r, X, ldx, Y, ldy, Z, ldz, acc, ldacc)) {
         constexpr int VW = 16 / sizeof(T);
-        constexpr int NCHMAX = sizeof(T) == 4 ? 3 : 5;  // 64-lane spans per wave
+        // 64-lane spans per wave; f64 with b = 1 stops at 3, where the
+        // compiler's hoisting of the unrolled loads would cut occupancy to 2
+        constexpr int NCHMAX = sizeof(T) == 4 ? 3 : 5;
+        const int nchmax = (sizeof(T) == 8 && b == 1) ? 3 : NCHMAX;
         const int spans = (int)cdiv(r / VW, 64);
-        const int groups = (spans + NCHMAX - 1) / NCHMAX;
+        const int groups = (spans + nchmax - 1) / nchmax;
         const int nch = (spans + groups - 1) / groups;
         dim3 g(8 * cdiv(cdiv(nb, 4), 8), groups);
 #define RVGP_VEC(BB, NN)                                                                           \

@@ -10,7 +10,12 @@ def _t(x, dt):
     return torch.as_tensor(np.ascontiguousarray(x), device="cuda", dtype=dt)
 
 
-@pytest.mark.parametrize("M,N,K", [(70, 33, 45), (64, 64, 16), (1, 5, 3), (130, 200, 257)])
+# shapes on both sides of the small-GEMM rule (K <= 1024 and few 128-tiles):
+# matrix-vector (500, 1, 500), a Cholesky-update shape (436, 436, 64), and
+# K = 1000 / 1100 around the threshold
+@pytest.mark.parametrize("M,N,K", [(70, 33, 45), (64, 64, 16), (1, 5, 3), (130, 200, 257),
+                                   (500, 1, 500), (70, 1, 3000), (436, 436, 64), (96, 300, 1000),
+                                   (96, 300, 1100)])
 @pytest.mark.parametrize("tA", [False, True])
 @pytest.mark.parametrize("tB", [False, True])
 def test_gemm_modes(M, N, K, tA, tB):

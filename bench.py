@@ -95,7 +95,7 @@ def traffic_from_profile():
     """HBM bytes per launch of the dominant kernel from the committed
     rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, scaled by a plain copy
     kernel), or None when no such summary is present."""
-    path = os.path.join(ROOT, "profiles", "spmm_wide_f32_b3_c3_pmc.txt")
+    path = os.path.join(ROOT, "profiles", "spmm_f32_b3_pmc.txt")
     if not os.path.exists(path):
         return None
     for line in open(path):

@@ -210,7 +210,7 @@ int rvgp_trsm(int f64, int trans, int n, int r, const void* L, int lda, void* B,
 int rvgp_chol_logdet(int f64, int n, const void* L, int lda, double* out, void* stream);
 
 /* out[j] = sum_i Z[i][j]^2 (f64 accumulation, fixed order); Z (M, k) f64/f32. */
-size_t rvgp_colsumsq_ws_bytes(int k);
+size_t rvgp_colsumsq_ws_bytes(int M, int k);
 int rvgp_colsumsq(int f64, int M, int k, const void* Z, int ldz, double* out, void* ws,
                   size_t ws_bytes, void* stream);
 

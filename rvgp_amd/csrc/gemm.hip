@@ -20,6 +20,11 @@
 //          (BF16 tiles with FP32 accumulation)
 #include "common.h"
 
+// The library builds with -ffp-contract=off (the distance kernels need a
+// fixed operation order); the epilogues and the matrix-vector kernels use FMAs; results are compared
+// with tolerances, never bit for bit.
+#pragma clang fp contract(fast)
+
 namespace rvgp {
 
 constexpr int BM = 64, BN = 64, GT = 256;

@@ -107,29 +107,44 @@ __global__ void ws_form_kernel(int k, const double* __restrict__ G, int ldg,
 }
 
 // Column sums of squares: out[j] = sum_i Z[i][j]^2, f64 accumulation, in a
-// fixed order: kColParts row slices write partials, a second pass sums them.
-constexpr int kColParts = 64;
+// fixed order: P(M) row slices of >= 256 rows (P <= 1024) write partials --
+// lane = column, so every row is read coalesced, four interleaved
+// accumulators per lane -- and a second pass sums them in slice order.
+constexpr int kColMaxParts = 1024;
+static inline int colsumsq_parts(int M) {
+    const int p = (M + 255) / 256;
+    return p < 1 ? 1 : (p > kColMaxParts ? kColMaxParts : p);
+}
+
 template <typename T>
-__global__ void colsumsq_part_kernel(int M, int k, const T* __restrict__ Z, int ldz,
+__global__ void colsumsq_part_kernel(int M, int k, int parts, const T* __restrict__ Z, int ldz,
                                      double* __restrict__ part) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= k) return;
-    const int per = (M + kColParts - 1) / kColParts;
+    const int per = (M + parts - 1) / parts;
     const int i0 = blockIdx.y * per, i1 = min(M, i0 + per);
-    double s = 0.0;
-    for (int i = i0; i < i1; ++i) {
-        const double z = (double)Z[(size_t)i * ldz + j];
-        s += z * z;
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    int i = i0;
+    for (; i + 3 < i1; i += 4) {
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const double z = (double)Z[(size_t)(i + t) * ldz + j];
+            s[t] += z * z;
+        }
     }
-    part[(size_t)blockIdx.y * k + j] = s;
+    for (; i < i1; ++i) {
+        const double z = (double)Z[(size_t)i * ldz + j];
+        s[0] += z * z;
+    }
+    part[(size_t)blockIdx.y * k + j] = (s[0] + s[1]) + (s[2] + s[3]);
 }
 
-__global__ void colsumsq_reduce_kernel(int k, const double* __restrict__ part,
+__global__ void colsumsq_reduce_kernel(int k, int parts, const double* __restrict__ part,
                                        double* __restrict__ out) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= k) return;
     double s = 0.0;
-    for (int p = 0; p < kColParts; ++p) s += part[(size_t)p * k + j];
+    for (int p = 0; p < parts; ++p) s += part[(size_t)p * k + j];
     out[j] = s;
 }
 
@@ -288,23 +303,26 @@ extern "C" int rvgp_ws_form(int k, const double* G, int ldg, const double* S, co
     return RVGP_OK;
 }
 
-extern "C" size_t rvgp_colsumsq_ws_bytes(int k) { return sizeof(double) * (size_t)kColParts * k; }
+extern "C" size_t rvgp_colsumsq_ws_bytes(int M, int k) {
+    return sizeof(double) * (size_t)colsumsq_parts(M) * (k > 0 ? k : 0);
+}
 
 extern "C" int rvgp_colsumsq(int f64, int M, int k, const void* Z, int ldz, double* out, void* ws,
                              size_t ws_bytes, void* stream) {
     RVGP_CHECK_ARG(M >= 0 && k >= 1 && ldz >= k, "rvgp_colsumsq: bad sizes");
-    RVGP_CHECK_ARG(ws_bytes >= rvgp_colsumsq_ws_bytes(k), "rvgp_colsumsq: workspace too small");
+    RVGP_CHECK_ARG(ws_bytes >= rvgp_colsumsq_ws_bytes(M, k), "rvgp_colsumsq: workspace too small");
     double* part = static_cast<double*>(ws);
-    dim3 g(cdiv(k, 256), kColParts);
+    const int parts = colsumsq_parts(M);
+    dim3 g(cdiv(k, 256), parts);
     if (f64)
         hipLaunchKernelGGL(colsumsq_part_kernel<double>, g, dim3(256), 0, as_stream(stream), M, k,
-                           static_cast<const double*>(Z), ldz, part);
+                           parts, static_cast<const double*>(Z), ldz, part);
     else
         hipLaunchKernelGGL(colsumsq_part_kernel<float>, g, dim3(256), 0, as_stream(stream), M, k,
-                           static_cast<const float*>(Z), ldz, part);
+                           parts, static_cast<const float*>(Z), ldz, part);
     RVGP_LAUNCH_CHECK();
     hipLaunchKernelGGL(colsumsq_reduce_kernel, dim3(cdiv(k, 256)), dim3(256), 0, as_stream(stream), k,
-                       part, out);
+                       parts, part, out);
     RVGP_LAUNCH_CHECK();
     return RVGP_OK;
 }

@@ -10,6 +10,11 @@
 // GEMMs, the diagonal-block solves one wavefront per right-hand side.
 #include "common.h"
 
+// The library builds with -ffp-contract=off (the distance kernels need a
+// fixed operation order); the substitutions accumulate with FMAs; results are compared with
+// tolerances, never bit for bit.
+#pragma clang fp contract(fast)
+
 extern "C" int rvgp_gemm(int mode, int transA, int transB, int M, int N, int K, double alpha,
                          const void* A, int lda, const void* B, int ldb, const double* kscale,
                          double beta, void* C, int ldc, int split_k, void* ws, size_t ws_bytes,

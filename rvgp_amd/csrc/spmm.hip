@@ -14,9 +14,9 @@
 //  * narrow (r <= 4): LPR lanes share one block row and split its blocks
 //    (nnz-parallel), reducing with cross-lane shuffles -- consecutive lanes
 //    read consecutive blocks, so block values stream coalesced.
-//  * vector (r > 4, 16-byte aligned rows): one wavefront per block row and
-//    up to 3 (f32) / 5 (f64) spans of 64 16-byte vectors; each X row segment
-//    is one 1 KiB coalesced load.
+//  * vector (r > 4, 16-byte aligned rows): one wavefront per (block row,
+//    column group of <= 64 16-byte vectors); each X row segment is one
+//    coalesced load of up to 1 KiB.
 //  * wide (r > 4, otherwise): one wavefront per (block row, 64*CPL-column
 //    chunk), one 4/8-byte column per lane per load.
 // In the wide forms the block row index is wave-uniform: the row's column
@@ -25,6 +25,11 @@
 // rows of one chunk run together and the chunk of X they gather stays in
 // cache.
 #include "common.h"
+
+// The library builds with -ffp-contract=off (the distance kernels need a
+// fixed operation order); products here are accumulated as FMAs (one VALU op per multiply-add);
+// nothing in this file is compared bit for bit with the oracle.
+#pragma clang fp contract(fast)
 
 namespace rvgp {
 
@@ -91,10 +96,14 @@ __global__ __launch_bounds__(256) void spmm_narrow(
 
 // Wide SpMM, vector form (16-byte rows).  When r, every leading dimension and
 // every base pointer are multiples of the vector width VW (4 f32 / 2 f64),
-// lane l owns the VW consecutive columns VW*l + 64*VW*t of its chunk, so one
-// gather instruction moves 1 KiB of an X row instead of 256 B, and one wave
-// covers up to NCH*64*VW columns of its block row.  Block values and column
-// indices are loaded and broadcast as in spmm_wide2.
+// the r/VW vectors of a row are cut into column groups of gv <= 64 vectors;
+// lane l of the wave for (block row, group g) owns vector g*gv + l, so one
+// gather instruction moves up to 1 KiB of an X row.  Groups are the slow grid
+// dimension: one group of every block row is swept before the next, which
+// keeps the neighbour rows the in-flight rows of an XCD gather within its
+// 4 MB L2 (a full-width sweep of r = 576 f32 columns needs ~6 KB per
+// neighbour row and overflowed it, re-fetching X from HBM ~4x).  Block values
+// and column indices are loaded and broadcast as in spmm_wide2.
 template <typename T, int VW>
 struct VecOf;
 template <>
@@ -106,16 +115,15 @@ struct VecOf<double, 2> {
     typedef double type __attribute__((ext_vector_type(2)));
 };
 
-template <typename T, int B, int VW, int NCH>
+template <typename T, int B, int VW>
 __global__ __launch_bounds__(256) void spmm_vec(
     int nb, const int* __restrict__ rowptr, const int* __restrict__ cols,
-    const T* __restrict__ vals, const T* __restrict__ X, int ldx, int r, T alpha, T beta,
+    const T* __restrict__ vals, const T* __restrict__ X, int ldx, int r, int gv, T alpha, T beta,
     const T* __restrict__ Z, int ldz, T gamma, T* __restrict__ Y, int ldy, T* __restrict__ acc,
     int ldacc, T delta) {
     using V = typename VecOf<T, VW>::type;
     constexpr int WPB = 4;
     constexpr int BB = B * B;
-    constexpr int SPAN = 64 * VW;
     const int lane = threadIdx.x & 63;
     // XCD-aware mapping: workgroups are dealt round-robin to the 8 XCDs, so
     // workgroup w runs on XCD w % 8; give each XCD one contiguous range of
@@ -125,12 +133,11 @@ __global__ __launch_bounds__(256) void spmm_vec(
     const int lw = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
     const int row = __builtin_amdgcn_readfirstlane(lw * WPB + (threadIdx.x >> 6));
     if (row >= nb) return;
-    const int cb = blockIdx.y * (SPAN * NCH) + VW * lane;
-    V y[B][NCH];
+    const int c = (blockIdx.y * gv + lane) * VW;
+    const bool on = lane < gv && c < r;
+    V y[B];
 #pragma unroll
-    for (int a = 0; a < B; ++a)
-#pragma unroll
-        for (int t = 0; t < NCH; ++t) y[a][t] = V(0);
+    for (int a = 0; a < B; ++a) y[a] = V(0);
     const int e0 = rowptr[row], e1 = rowptr[row + 1];
     for (int base = e0; base < e1; base += 64) {
         const int cnt = min(64, e1 - base);
@@ -153,41 +160,97 @@ __global__ __launch_bounds__(256) void spmm_vec(
                         const int f = e * BB + a * B + bb;  // constant after unrolling
                         v[a][bb] = bcast(mv[f >> 6], f & 63);
                     }
-                const T* xr = X + (size_t)j * B * ldx;
+                if (on) {
+                    const T* xr = X + (size_t)j * B * ldx + c;
+                    V xb[B];
 #pragma unroll
-                for (int t = 0; t < NCH; ++t) {
-                    const int c = cb + SPAN * t;
-                    if (c < r) {
-                        V xb[B];
+                    for (int bb = 0; bb < B; ++bb)
+                        xb[bb] = *reinterpret_cast<const V*>(xr + (size_t)bb * ldx);
 #pragma unroll
-                        for (int bb = 0; bb < B; ++bb)
-                            xb[bb] = *reinterpret_cast<const V*>(xr + (size_t)bb * ldx + c);
+                    for (int a = 0; a < B; ++a)
 #pragma unroll
-                        for (int a = 0; a < B; ++a)
-#pragma unroll
-                            for (int bb = 0; bb < B; ++bb) y[a][t] += v[a][bb] * xb[bb];
-                    }
+                        for (int bb = 0; bb < B; ++bb) y[a] += v[a][bb] * xb[bb];
                 }
             }
         }
     }
+    if (!on) return;
 #pragma unroll
     for (int a = 0; a < B; ++a) {
         const size_t rr = (size_t)(row * B + a);
+        V out = alpha * y[a];
+        if (beta != T(0)) out += beta * *reinterpret_cast<const V*>(X + rr * ldx + c);
+        if (Z != nullptr && gamma != T(0))
+            out += gamma * *reinterpret_cast<const V*>(Z + rr * ldz + c);
+        *reinterpret_cast<V*>(Y + rr * ldy + c) = out;
+        if (acc != nullptr) {
+            V* ap = reinterpret_cast<V*>(acc + rr * ldacc + c);
+            *ap = *ap + delta * out;
+        }
+    }
+}
+
+// Wide SpMM, buffer form: the vector form with the gather address path moved
+// off the vector ALU.  Column indices and block values are read with scalar
+// loads (wave-uniform addresses) straight into SGPRs that feed the FMAs, and
+// X is gathered with raw buffer loads: the per-lane column offset is a fixed
+// VGPR, the neighbour row's byte offset a scalar -- so per stored block a
+// lane issues B 16-byte loads and B*B*VW FMAs and nothing else.  Lanes past
+// the row's columns point their offset past the buffer, which the hardware
+// range check turns into a zero load without a memory access.  Needs X to
+// span < 2 GiB (32-bit offsets).
+constexpr int kBufRsrcWord3 = 0x00020000;  // raw 32-bit dwords, gfx9 family
+
+template <typename T, int B, int VW>
+__global__ __launch_bounds__(256) void spmm_buf(
+    int nb, const int* __restrict__ rowptr, const int* __restrict__ cols,
+    const T* __restrict__ vals, const T* __restrict__ X, int ldx, int r, int gv, int xbytes,
+    T alpha, T beta, const T* __restrict__ Z, int ldz, T gamma, T* __restrict__ Y, int ldy,
+    T* __restrict__ acc, int ldacc, T delta) {
+    using V = typename VecOf<T, VW>::type;
+    constexpr int WPB = 4;
+    constexpr int BB = B * B;
+    const int lane = threadIdx.x & 63;
+    const int per = gridDim.x >> 3;  // XCD-aware row ranges, as in spmm_vec
+    const int lw = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    const int row = __builtin_amdgcn_readfirstlane(lw * WPB + (threadIdx.x >> 6));
+    if (row >= nb) return;
+    const int c = (blockIdx.y * gv + lane) * VW;
+    const bool on = lane < gv && c < r;
+    const __amdgpu_buffer_rsrc_t xr =
+        __builtin_amdgcn_make_buffer_rsrc((void*)X, (short)0, xbytes, kBufRsrcWord3);
+    const int voff = on ? c * (int)sizeof(T) : xbytes;
+    const int rowbytes = ldx * (int)sizeof(T);
+    V y[B];
 #pragma unroll
-        for (int t = 0; t < NCH; ++t) {
-            const int c = cb + SPAN * t;
-            if (c < r) {
-                V out = alpha * y[a][t];
-                if (beta != T(0)) out += beta * *reinterpret_cast<const V*>(X + rr * ldx + c);
-                if (Z != nullptr && gamma != T(0))
-                    out += gamma * *reinterpret_cast<const V*>(Z + rr * ldz + c);
-                *reinterpret_cast<V*>(Y + rr * ldy + c) = out;
-                if (acc != nullptr) {
-                    V* ap = reinterpret_cast<V*>(acc + rr * ldacc + c);
-                    *ap = *ap + delta * out;
-                }
-            }
+    for (int a = 0; a < B; ++a) y[a] = V(0);
+    const int e0 = rowptr[row], e1 = rowptr[row + 1];
+#pragma unroll 4
+    for (int e = e0; e < e1; ++e) {
+        const int j = cols[e];
+        const T* blk = vals + (size_t)e * BB;
+        V xb[B];
+#pragma unroll
+        for (int bb = 0; bb < B; ++bb)
+            xb[bb] = __builtin_bit_cast(
+                V, __builtin_amdgcn_raw_buffer_load_b128(xr, voff, (j * B + bb) * rowbytes, 0));
+#pragma unroll
+        for (int a = 0; a < B; ++a)
+#pragma unroll
+            for (int bb = 0; bb < B; ++bb) y[a] += blk[a * B + bb] * xb[bb];
+    }
+    if (!on) return;
+#pragma unroll
+    for (int a = 0; a < B; ++a) {
+        const size_t rr = (size_t)(row * B + a);
+        V out = alpha * y[a];
+        if (beta != T(0)) out += beta * *reinterpret_cast<const V*>(X + rr * ldx + c);
+        if (Z != nullptr && gamma != T(0))
+            out += gamma * *reinterpret_cast<const V*>(Z + rr * ldz + c);
+        *reinterpret_cast<V*>(Y + rr * ldy + c) = out;
+        if (acc != nullptr) {
+            V* ap = reinterpret_cast<V*>(acc + rr * ldacc + c);
+            *ap = *ap + delta * out;
         }
     }
 }
@@ -310,29 +373,29 @@ static int spmm_dispatch(int nb, int b, const int* rowptr, const int* cols, cons
 #undef RVGP_NARROW
     } else if (vec_ok<T>(r, X, ldx, Y, ldy, Z, ldz, acc, ldacc)) {
         constexpr int VW = 16 / sizeof(T);
-        // 64-lane spans per wave; f64 with b = 1 stops at 3, where the
-        // compiler's hoisting of the unrolled loads would cut occupancy to 2
-        constexpr int NCHMAX = sizeof(T) == 4 ? 3 : 5;
-        const int nchmax = (sizeof(T) == 8 && b == 1) ? 3 : NCHMAX;
-        const int spans = (int)cdiv(r / VW, 64);
-        const int groups = (spans + nchmax - 1) / nchmax;
-        const int nch = (spans + groups - 1) / groups;
+        const int nv = r / VW;                        // 16-byte vectors per row
+        const int groups = (int)cdiv(nv, 64);         // column groups of <= 64 vectors
+        const int gv = (nv + groups - 1) / groups;    // balanced group width
         dim3 g(8 * cdiv(cdiv(nb, 4), 8), groups);
-#define RVGP_VEC(BB, NN)                                                                           \
-    hipLaunchKernelGGL((spmm_vec<T, BB, VW, NN>), g, dim3(256), 0, s, nb, rowptr, cols, vals, X, \
-                       ldx, r, alpha, beta, Z, ldz, gamma, Y, ldy, acc, ldacc, delta)
-#define RVGP_VEC_B(NN)                 \
-    if (b == 1) RVGP_VEC(1, NN);       \
-    else if (b == 2) RVGP_VEC(2, NN);  \
-    else RVGP_VEC(3, NN);
-        if (nch == 1) { RVGP_VEC_B(1) }
-        else if (nch == 2) { RVGP_VEC_B(2) }
-        else if (nch == 3) { RVGP_VEC_B(3) }
-        else if constexpr (NCHMAX >= 5) {
-            if (nch == 4) { RVGP_VEC_B(4) }
-            else { RVGP_VEC_B(5) }
+        const size_t xbytes = (size_t)nb * b * ldx * sizeof(T);
+        if (xbytes < ((size_t)1 << 31)) {
+#define RVGP_BUF(BB)                                                                          \
+    hipLaunchKernelGGL((spmm_buf<T, BB, VW>), g, dim3(256), 0, s, nb, rowptr, cols, vals, X, \
+                       ldx, r, gv, (int)xbytes, alpha, beta, Z, ldz, gamma, Y, ldy, acc,    \
+                       ldacc, delta)
+            if (b == 1) RVGP_BUF(1);
+            else if (b == 2) RVGP_BUF(2);
+            else RVGP_BUF(3);
+#undef RVGP_BUF
+            RVGP_LAUNCH_CHECK();
+            return RVGP_OK;
         }
-#undef RVGP_VEC_B
+#define RVGP_VEC(BB)                                                                          \
+    hipLaunchKernelGGL((spmm_vec<T, BB, VW>), g, dim3(256), 0, s, nb, rowptr, cols, vals, X, \
+                       ldx, r, gv, alpha, beta, Z, ldz, gamma, Y, ldy, acc, ldacc, delta)
+        if (b == 1) RVGP_VEC(1);
+        else if (b == 2) RVGP_VEC(2);
+        else RVGP_VEC(3);
 #undef RVGP_VEC
     } else {
         // columns per lane: balance the column chunks (at most 4 f32 / 2 f64)

@@ -85,7 +85,7 @@ def colsumsq(Z):
     """out[j] = sum_i Z[i, j]^2 in FP64 (deterministic order), Z f32 or f64."""
     k = Z.shape[1]
     out = torch.empty(k, dtype=torch.float64, device=Z.device)
-    ws_bytes = _lib.lib().rvgp_colsumsq_ws_bytes(k)
+    ws_bytes = _lib.lib().rvgp_colsumsq_ws_bytes(Z.shape[0], k)
     ws = torch.empty(ws_bytes, dtype=torch.uint8, device=Z.device)
     _lib.call("rvgp_colsumsq", int(Z.dtype == torch.float64), Z.shape[0], k, _lib.ptr(Z), _ld(Z),
               _lib.ptr(out), _lib.ptr(ws), ws_bytes, _lib.stream())

@@ -105,18 +105,23 @@ def _cheb_filter(A, X, m, a, b, a0):
     return cur
 
 
-def _degree(theta, k, cut, ub, max_ratio, m_min=4, m_max=60):
+def _degree(theta, k, cut, ub, max_ratio, m_min=4, m_max=60, need=None):
     """Largest degree whose amplification spread over the block stays below
-    max_ratio (keeps the filtered block well enough conditioned)."""
+    max_ratio (keeps the filtered block well enough conditioned), and no
+    larger than the Chebyshev estimate of what the worst wanted residual still
+    needs: a degree-m filter damps the unwanted end against the k-th Ritz
+    value by ~exp(-m acosh(x_k)), so m = ln(need) / acosh(x_k) (need = the
+    residual reduction still required, with a safety factor applied by the
+    caller)."""
     e = (ub - cut) / 2.0
     c = (ub + cut) / 2.0
     x0 = (c - float(theta[0])) / e
     xk = (c - float(theta[min(k, len(theta)) - 1])) / e
     a0 = math.acosh(max(x0, 1.0 + 1e-15))
     ak = math.acosh(max(xk, 1.0 + 1e-15))
-    if a0 - ak <= 1e-12:
-        return m_max
-    m = int(math.log(max_ratio) / (a0 - ak))
+    m = m_max if a0 - ak <= 1e-12 else int(math.log(max_ratio) / (a0 - ak))
+    if need is not None and need > 1.0 and ak > 1e-12:
+        m = min(m, int(math.ceil(math.log(need) / ak)))
     return max(m_min, min(m_max, m))
 
 
@@ -185,7 +190,8 @@ def _run_phase(A, X, k, ub, wdt, ptol, max_iter, max_ratio, lock, fresh, stats, 
         cut = float(tha[min(tha.numel(), Xa.shape[1]) - 1].item())
         low = float(tha[0].item())
         kk = min(k - Xl.shape[1], tha.numel())
-        m = _degree(tha, max(kk, 1), cut, ub, max_ratio, m_max=m_max)
+        m = _degree(tha, max(kk, 1), cut, ub, max_ratio, m_max=m_max,
+                    need=10.0 * worst / ptol if kk > 0 else None)
         tf = time.perf_counter()
         Y = _cheb_filter(A, Xa, m, cut, ub, low)
         torch.cuda.synchronize()
@@ -219,8 +225,10 @@ def chfsi(A: BSR, k, precision="fp64", tol=1e-9, max_iter=300, guard=None, seed=
     t0 = time.perf_counter()
     X = X0.contiguous() if X0 is not None else fresh(p, torch.float64)
     if precision == "mixed":
-        # FP32 filter body with locking, down to the FP32 residual floor
-        X, theta, it = _run_phase(A, X.to(torch.float32).contiguous(), k, ub, torch.float32, 3e-6,
+        # FP32 filter body with locking; hands over at 1e-5, a few times above
+        # the FP32 residual floor (a phase that reaches the floor first
+        # stalls there)
+        X, theta, it = _run_phase(A, X.to(torch.float32).contiguous(), k, ub, torch.float32, 1e-5,
                                   max_iter, 300.0, True, fresh, stats, verbose, "fp32",
                                   stall_exit=True)
         stats["iters_fp32"] = it

@@ -60,7 +60,8 @@ class GPREngine:
         self.gS = torch.empty(self.k, **f64)
         self.out = torch.zeros(9, **f64)
         self.status = torch.zeros(1, dtype=torch.int32, device=dev)
-        self._cs_ws = torch.empty(_lib.lib().rvgp_colsumsq_ws_bytes(max(self.k, 1)) // 8 + 1, **f64)
+        self._cs_ws = torch.empty(
+            _lib.lib().rvgp_colsumsq_ws_bytes(max(self.M, self.k), max(self.k, 1)) // 8 + 1, **f64)
         if self.form == "weight" or precision == "mixed":
             # k x k normal-equation system (weight space; in the mixed
             # function-space form it carries log det and diag(Phi^T K^-1 Phi)
@@ -242,8 +243,8 @@ class GPREngine:
             Zt = Ps.T.contiguous()
             dense.trsm(self.A, Zt)
             v = torch.empty(m, dtype=torch.float64, device=self.dev)
-            cs_ws = torch.empty(_lib.lib().rvgp_colsumsq_ws_bytes(m) // 8 + 1, dtype=torch.float64,
-                                device=self.dev)
+            cs_ws = torch.empty(_lib.lib().rvgp_colsumsq_ws_bytes(self.k, m) // 8 + 1,
+                                dtype=torch.float64, device=self.dev)
             _lib.call("rvgp_colsumsq", 1, self.k, m, _lib.ptr(Zt), m, _lib.ptr(v), _lib.ptr(cs_ws),
                       cs_ws.numel() * 8, _lib.stream())
             var = self.s2 * v
@@ -252,8 +253,8 @@ class GPREngine:
             mean = dense.gemm(Ps, (p * self.S.reshape(-1, 1)).contiguous()).reshape(-1)
             Kms = dense.gemm(self.Phi, Ps, transB=True, kscale=self.S)  # (M, m*)
             self._solve_L(Kms)
-            cs_ws = torch.empty(_lib.lib().rvgp_colsumsq_ws_bytes(m) // 8 + 1, dtype=torch.float64,
-                                device=self.dev)
+            cs_ws = torch.empty(_lib.lib().rvgp_colsumsq_ws_bytes(max(self.M, self.k), m) // 8 + 1,
+                                dtype=torch.float64, device=self.dev)
             v = torch.empty(m, dtype=torch.float64, device=self.dev)
             _lib.call("rvgp_colsumsq", 1, self.M, m, _lib.ptr(Kms), m, _lib.ptr(v), _lib.ptr(cs_ws),
                       cs_ws.numel() * 8, _lib.stream())

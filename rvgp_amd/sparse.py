@@ -66,13 +66,14 @@ def spmm(A, X, Y=None, alpha=1.0, beta=0.0, Z=None, gamma=0.0, acc=None, delta=0
     Y2 = Y.reshape(-1, 1) if Y.dim() == 1 else Y
     Z2 = None if Z is None else (Z.reshape(-1, 1) if Z.dim() == 1 else Z)
     A2 = None if acc is None else (acc.reshape(-1, 1) if acc.dim() == 1 else acc)
-    name = "rvgp_bsr_spmm_f64" if X2.dtype == torch.float64 else "rvgp_bsr_spmm_f32"
+    f64 = X2.dtype == torch.float64
     vals = A.vals(X2.dtype)
     ev = instrument.begin()
-    _lib.call(name, A.nb, A.b, _lib.ptr(A.rowptr), _lib.ptr(A.cols), _lib.ptr(vals),
-              _lib.ptr(X2), X2.stride(0), r, float(alpha), float(beta), _lib.ptr(Z2),
-              Z2.stride(0) if Z2 is not None else r, float(gamma), _lib.ptr(Y2), Y2.stride(0),
-              _lib.ptr(A2), A2.stride(0) if A2 is not None else r, float(delta), _lib.stream())
+    _lib.call("rvgp_bsr_spmm_f64" if f64 else "rvgp_bsr_spmm_f32", A.nb, A.b,
+              _lib.ptr(A.rowptr), _lib.ptr(A.cols), _lib.ptr(vals), _lib.ptr(X2), X2.stride(0), r,
+              float(alpha), float(beta), _lib.ptr(Z2), Z2.stride(0) if Z2 is not None else r,
+              float(gamma), _lib.ptr(Y2), Y2.stride(0), _lib.ptr(A2),
+              A2.stride(0) if A2 is not None else r, float(delta), _lib.stream())
     if ev is not None:
         s = X2.element_size()
         nbytes = spmm_bytes(A, r, s, Z is not None and gamma != 0.0, acc is not None)

@@ -51,7 +51,8 @@ def test_workspace_queries():
     L = _lib.load_symbols_only()
     assert L.rvgp_graph_symmetrize_ws_bytes(100, 10) >= 4 * 100 * 21
     assert L.rvgp_geo_nbh_ws_bytes(2, 100) == 2 * L.rvgp_geo_nbh_ws_bytes(1, 100)
-    assert L.rvgp_colsumsq_ws_bytes(10) >= 8 * 10
+    assert L.rvgp_colsumsq_ws_bytes(1000, 10) >= 8 * 10
+    assert L.rvgp_colsumsq_ws_bytes(10**6, 10) == 8 * 10 * 1024  # at most 1024 row slices
 
 
 def test_host_split_rule_and_softplus():

@@ -214,7 +214,7 @@ def test_connections_vs_reference(case):
 
 # ---------------------------------------------------------------- SpMM
 @pytest.mark.parametrize("b", [1, 2, 3])
-@pytest.mark.parametrize("r", [1, 3, 7, 130])
+@pytest.mark.parametrize("r", [1, 3, 7, 130, 136])
 def test_bsr_spmm(b, r):
     import torch
     from rvgp_amd.sparse import BSR, spmm
@@ -241,3 +241,4 @@ def test_bsr_spmm(b, r):
         # plain product, in place over Z allowed
         Y2 = spmm(A, Xd, Y=Zd, alpha=1.0, Z=Zd, gamma=0.0)
         assert np.abs(Y2.cpu().double().numpy() - As @ X).max() < tol * np.abs(As @ X).max() * 10
+

@@ -163,8 +163,8 @@ def gpu_same_sample(args):
 
 def roofline_only(args):
     """Launch the dominant kernel alone at the headline shape: the FP32 block
-    SpMM (b = 3) of the Lc eigensolve on the Morton-renumbered operator, with
-    the Chebyshev-step epilogue (Z read) at the starting block width
+    SpMM (b = 3, buffer form) of the Lc eigensolve on the Morton-renumbered
+    operator, with the Chebyshev-step epilogue (Z read) at the starting block width
     r = k + guard rounded up to a multiple of 8,
     then a plain 1 GiB device copy (the reference for scaling PMC byte
     counters).  Used under rocprofv3 --pmc; prints one JSON line."""
@@ -205,7 +205,7 @@ def roofline_only(args):
     for _ in range(3):
         torch.mul(src, 1.0, out=dst)  # an elementwise kernel: reads 1 GiB, writes 1 GiB
     torch.cuda.synchronize()
-    print(json.dumps({"kernel": "spmm_vec<float,3,4,*>", "r": r, "nb": A.nb, "nnzb": A.nnzb,
+    print(json.dumps({"kernel": "spmm_buf<float,3,4>", "r": r, "nb": A.nb, "nnzb": A.nnzb,
                       "launches": reps + 2, "avg_us": us, "alg_bytes": nbytes,
                       "GBps": nbytes / (us * 1e3), "copy_bytes_each_way": 1 << 30}))
 
@@ -294,7 +294,8 @@ def main():
     # roofline of the dominant kernel: FP32 block SpMM (b = dim_man) of the
     # Lc eigensolve, measured over every launch of that kernel in the timed fit
     dm = d_mx.dim_man
-    cands = [f"spmm_vec[float32,b{dm}]", f"spmm_wide[float32,b{dm}]"]
+    cands = [f"spmm_buf[float32,b{dm}]", f"spmm_vec[float32,b{dm}]",
+             f"spmm_wide[float32,b{dm}]"]
     key = max(cands, key=lambda c: kstats.get(c, {}).get("total_ms", 0.0))
     ks = kstats.get(key, {"count": 0, "avg_us": 0.0, "GBps": 0.0})
     roof = {"kernel": key, "bound": "hbm", "achieved": round(ks["GBps"], 1), "peak": HBM_PEAK_GBS,

@@ -77,8 +77,12 @@ def spmm(A, X, Y=None, alpha=1.0, beta=0.0, Z=None, gamma=0.0, acc=None, delta=0
     if ev is not None:
         s = X2.element_size()
         nbytes = spmm_bytes(A, r, s, Z is not None and gamma != 0.0, acc is not None)
-        kind = "spmm_narrow" if r <= 4 else (
-            "spmm_vec" if _vec_ok(r, X2, Y2, Z2, A2) else "spmm_wide")
+        if r <= 4:
+            kind = "spmm_narrow"
+        elif not _vec_ok(r, X2, Y2, Z2, A2):
+            kind = "spmm_wide"
+        else:  # the buffer form addresses X with 32-bit offsets
+            kind = "spmm_buf" if A.nb * A.b * X2.stride(0) * s < 2 ** 31 else "spmm_vec"
         instrument.end(ev, kind, (str(X2.dtype).replace("torch.", ""), A.b), nbytes,
                        2.0 * A.nnzb * A.b * A.b * r)
     return Y

@@ -15,6 +15,12 @@
 // Singular-vector signs are free, as they are for LAPACK's dgesvd.
 #include "common.h"
 
+// The library builds with -ffp-contract=off; the Jacobi rotations here may
+// fuse multiply-adds (the centring and the Gram keep their explicit
+// __fma_rn order, and frames / singular values are compared with
+// tolerances).
+#pragma clang fp contract(fast)
+
 namespace rvgp {
 
 __global__ __launch_bounds__(64) void gauges_kernel(const double* __restrict__ x, int n, int D,

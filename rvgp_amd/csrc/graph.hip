@@ -81,7 +81,23 @@ __global__ __launch_bounds__(kKnnThreads) void knn_kernel(const double* __restri
         }
         __syncthreads();
         const int jend = min(TILE, s1 - j0);
-        for (int jj = 0; jj < jend; ++jj) {
+        // two candidates per pass: their DMAX-long FMA chains are independent
+        // and interleave; they are then offered in index order
+        int jj = 0;
+        for (; jj + 1 < jend; jj += 2) {
+            double acc0 = 0.0, acc1 = 0.0;
+#pragma unroll
+            for (int c = 0; c < DMAX; ++c) {
+                const double t0 = __dsub_rn(q[c], tile[jj * DMAX + c]);
+                const double t1 = __dsub_rn(q[c], tile[(jj + 1) * DMAX + c]);
+                acc0 = __fma_rn(t0, t0, acc0);
+                acc1 = __fma_rn(t1, t1, acc1);
+            }
+            const int j = j0 + jj;
+            if (j != i && acc0 < bd[KMAX - 1]) knn_insert<KMAX>(bd, bi, acc0, j);
+            if (j + 1 != i && acc1 < bd[KMAX - 1]) knn_insert<KMAX>(bd, bi, acc1, j + 1);
+        }
+        if (jj < jend) {
             const int j = j0 + jj;
             double acc = 0.0;
 #pragma unroll

@@ -193,6 +193,14 @@ int rvgp_gemm(int mode, int transA, int transB, int M, int N, int K, double alph
 /* Workspace bytes rvgp_gemm needs for this shape; split_k <= 0 lets the
  * library choose the split (from the shape only, so results are
  * reproducible).  0 when no split is used. */
+/* Symmetric product C = A^T B (M x M, A and B K x M row-major, f64 out) for
+ * Grams known to be symmetric (A^T A, or A^T (S A) with S symmetric): only
+ * the upper 128 x 128 tiles are computed, the lower triangle is mirrored from
+ * the upper.  mode F64 or F32_ACC64; split_k / ws as rvgp_gemm (workspace
+ * from rvgp_gemm_ws_bytes(mode, M, M, K, split_k)).  The Rayleigh-Ritz Grams
+ * of the eigensolver that replaces eigsh (geometry.py:73). */
+int rvgp_gemm_sym(int mode, int M, int K, const void* A, int lda, const void* B, int ldb,
+                  double* C, int ldc, int split_k, void* ws, size_t ws_bytes, void* stream);
 size_t rvgp_gemm_ws_bytes(int mode, int M, int N, int K, int split_k);
 
 /* In-place lower Cholesky A = L L^T of an n x n row-major matrix (f64 or

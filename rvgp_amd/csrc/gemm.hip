@@ -81,7 +81,8 @@ __global__ __launch_bounds__(GT) void gemm_kernel(int transA, int transB, int M,
                                                   int ldb, const double* __restrict__ kscale,
                                                   typename Tr<MD>::Acc beta,
                                                   typename Tr<MD>::Acc* __restrict__ C, int ldc,
-                                                  typename Tr<MD>::Acc* __restrict__ part) {
+                                                  typename Tr<MD>::Acc* __restrict__ part,
+                                                  int upper) {
     using In = typename Tr<MD>::In;
     using Lds = typename Tr<MD>::Lds;
     using Acc = typename Tr<MD>::Acc;
@@ -91,6 +92,7 @@ __global__ __launch_bounds__(GT) void gemm_kernel(int transA, int transB, int M,
     __shared__ Lds As[BK][BM + PAD];
     __shared__ Lds Bs[BK][BN + PAD];
 
+    if (upper && blockIdx.y < blockIdx.x) return;  // strictly-lower tile of a symmetric C
     const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
     const int kbeg = blockIdx.z * kchunk;
     const int kend = min(K, kbeg + kchunk);
@@ -209,7 +211,8 @@ __global__ __launch_bounds__(GT) __attribute__((amdgpu_waves_per_eu(2))) void ge
                                                       int ldb, const double* __restrict__ kscale,
                                                       typename Tr<MD>::Acc beta,
                                                       typename Tr<MD>::Acc* __restrict__ C, int ldc,
-                                                      typename Tr<MD>::Acc* __restrict__ part) {
+                                                      typename Tr<MD>::Acc* __restrict__ part,
+                                                      int upper) {
     using In = typename Tr<MD>::In;
     using Lds = typename Tr<MD>::Lds;
     using Acc = typename Tr<MD>::Acc;
@@ -224,6 +227,7 @@ __global__ __launch_bounds__(GT) __attribute__((amdgpu_waves_per_eu(2))) void ge
     __shared__ Lds As[BK][BBM + PAD];
     __shared__ Lds Bs[BK][BBN + PAD];
 
+    if (upper && blockIdx.y < blockIdx.x) return;  // strictly-lower tile of a symmetric C
     const int m0 = blockIdx.x * BBM, n0 = blockIdx.y * BBN;
     const int kbeg = blockIdx.z * kchunk;
     const int kend = min(K, kbeg + kchunk);
@@ -487,6 +491,16 @@ __global__ __launch_bounds__(256) void gemv_kernel(int transA, int M, int K,
     }
 }
 
+// C[i][j] = C[j][i] for i > j: completes a symmetric C whose upper tiles
+// were computed.
+template <typename Acc>
+__global__ void mirror_lower(int n, Acc* __restrict__ C, int ldc) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)n * n) return;
+    const int i = (int)(e / n), j = (int)(e - (size_t)i * n);
+    if (i > j) C[(size_t)i * ldc + j] = C[(size_t)j * ldc + i];
+}
+
 struct GemmPlan {
     bool small, big;
     int split, kchunk;
@@ -528,7 +542,7 @@ template <Mode MD>
 static int gemm_launch(int transA, int transB, int M, int N, int K, double alpha, const void* A,
                        int lda, const void* B, int ldb, const double* kscale, double beta,
                        void* C, int ldc, int split, void* ws, size_t ws_bytes, void* stream,
-                       int lower = 0) {
+                       int lower = 0, int upper = 0) {
     using In = typename Tr<MD>::In;
     using Acc = typename Tr<MD>::Acc;
     RVGP_CHECK_ARG(M >= 0 && N >= 0 && K >= 0, "rvgp_gemm: negative size");
@@ -541,7 +555,7 @@ static int gemm_launch(int transA, int transB, int M, int N, int K, double alpha
                          : MD == Mode::F32ACC64 ? RVGP_GEMM_F32_ACC64 : RVGP_GEMM_BF16;
     const GemmPlan pl = gemm_plan(MODE, M, N, K, Tr<MD>::BK, split);
     if constexpr (MD != Mode::BF16) {
-        if (N == 1 && K <= 8192 && kscale == nullptr && !lower && split <= 1) {
+        if (N == 1 && K <= 8192 && kscale == nullptr && !lower && !upper && split <= 1) {
             const int incx = transB ? 1 : ldb;
             if (transA)
                 hipLaunchKernelGGL(gemv_kernel<MD>, dim3(cdiv(M, 64)), dim3(256), 0, s, 1, M, K,
@@ -557,7 +571,7 @@ static int gemm_launch(int transA, int transB, int M, int N, int K, double alpha
             return RVGP_OK;
         }
     }
-    if (pl.small) {
+    if (pl.small && !upper) {
         if constexpr (MD != Mode::BF16) {
             const int tn = (int)cdiv(N, 16);
             const size_t tiles = (size_t)cdiv(M, 16) * tn;
@@ -581,17 +595,22 @@ static int gemm_launch(int transA, int transB, int M, int N, int K, double alpha
         hipLaunchKernelGGL(gemm_big_kernel<MD>, grid, dim3(GT), 0, s, transA, transB, M, N, K,
                            pl.kchunk, (Acc)alpha, static_cast<const In*>(A), lda,
                            static_cast<const In*>(B), ldb, kscale, (Acc)beta, static_cast<Acc*>(C),
-                           ldc, part);
+                           ldc, part, upper);
     } else {
         dim3 grid(cdiv(M, BM), cdiv(N, BN), pl.split);
         hipLaunchKernelGGL(gemm_kernel<MD>, grid, dim3(GT), 0, s, transA, transB, M, N, K, pl.kchunk,
                            (Acc)alpha, static_cast<const In*>(A), lda, static_cast<const In*>(B), ldb,
-                           kscale, (Acc)beta, static_cast<Acc*>(C), ldc, part);
+                           kscale, (Acc)beta, static_cast<Acc*>(C), ldc, part, upper);
     }
     RVGP_LAUNCH_CHECK();
     if (pl.split > 1) {
         hipLaunchKernelGGL(splitk_reduce<Acc>, dim3(cdiv((size_t)M * N, 256)), dim3(256), 0, s, M, N,
                            pl.split, (Acc)alpha, part, (Acc)beta, static_cast<Acc*>(C), ldc);
+        RVGP_LAUNCH_CHECK();
+    }
+    if (upper) {
+        hipLaunchKernelGGL(mirror_lower<Acc>, dim3(cdiv((size_t)M * M, 256)), dim3(256), 0, s, M,
+                           static_cast<Acc*>(C), ldc);
         RVGP_LAUNCH_CHECK();
     }
     return RVGP_OK;
@@ -630,6 +649,18 @@ extern "C" size_t rvgp_gemm_ws_bytes(int mode, int M, int N, int K, int split_k)
     const size_t acc = (mode == RVGP_GEMM_F64 || mode == RVGP_GEMM_F32_ACC64) ? 8 : 4;
     const GemmPlan p = gemm_plan(mode, M, N, K, BK, split_k);
     return p.split > 1 ? acc * (size_t)p.split * M * N : 0;
+}
+
+extern "C" int rvgp_gemm_sym(int mode, int M, int K, const void* A, int lda, const void* B,
+                             int ldb, double* C, int ldc, int split_k, void* ws, size_t ws_bytes,
+                             void* stream) {
+    RVGP_CHECK_ARG(mode == RVGP_GEMM_F64 || mode == RVGP_GEMM_F32_ACC64,
+                   "rvgp_gemm_sym: mode must be F64 or F32_ACC64");
+    if (mode == RVGP_GEMM_F64)
+        return gemm_launch<Mode::F64>(1, 0, M, M, K, 1.0, A, lda, B, ldb, nullptr, 0.0, C, ldc,
+                                      split_k, ws, ws_bytes, stream, 0, 1);
+    return gemm_launch<Mode::F32ACC64>(1, 0, M, M, K, 1.0, A, lda, B, ldb, nullptr, 0.0, C, ldc,
+                                       split_k, ws, ws_bytes, stream, 0, 1);
 }
 
 namespace rvgp {

@@ -78,8 +78,11 @@ __global__ __launch_bounds__(64) void potrf_diag(T* __restrict__ A, int lda, int
 // One wavefront per right-hand side, lane a holding x_a: step a broadcasts
 // x_a = b_a / L_aa (v_readlane) and the lanes below (above, transposed)
 // subtract L_ja x_a -- a 64-step chain of readlane, multiply and FMA instead
-// of a serial 2016-FMA substitution per lane.  L11 and 1 / diag(L11) are
-// staged in LDS; 4 right-hand sides per workgroup.
+// of a serial 2016-FMA substitution per lane.  Each lane keeps the entries of
+// L11 it multiplies (its row, or its column when transposed) in registers,
+// so the unrolled chain touches no memory; entries outside the triangle are
+// zero, which leaves finished lanes unchanged without a predicate.  L11 is
+// staged once per workgroup in LDS; 4 right-hand sides per workgroup.
 template <typename T>
 __global__ __launch_bounds__(256) void trsm_diag(int trans, const T* __restrict__ Lm, int lda,
                                                  int i0, int nb, int r, T* __restrict__ B,
@@ -100,17 +103,27 @@ __global__ __launch_bounds__(256) void trsm_diag(int trans, const T* __restrict_
     if (c >= r) return;
     T* bc = B + (size_t)c * cs;
     T x = lane < nb ? bc[(size_t)lane * rs] : T(0);
+    const T dv = dinv[lane];
+    T l[NB];
     if (!trans) {
-        for (int a = 0; a < nb; ++a) {
-            const T xa = bcast(x, a) * dinv[a];
-            if (lane == a) x = xa;
-            else if (lane > a) x -= L[lane][a] * xa;
+#pragma unroll
+        for (int a = 0; a < NB; ++a) l[a] = L[lane][a];
+#pragma unroll
+        for (int a = 0; a < NB; ++a) {
+            if (a < nb) {
+                const T xa = bcast(x * dv, a);
+                x = lane == a ? xa : x - l[a] * xa;
+            }
         }
     } else {
-        for (int a = nb - 1; a >= 0; --a) {
-            const T xa = bcast(x, a) * dinv[a];
-            if (lane == a) x = xa;
-            else if (lane < a) x -= L[a][lane] * xa;
+#pragma unroll
+        for (int a = 0; a < NB; ++a) l[a] = L[a][lane];
+#pragma unroll
+        for (int a = NB - 1; a >= 0; --a) {
+            if (a < nb) {
+                const T xa = bcast(x * dv, a);
+                x = lane == a ? xa : x - l[a] * xa;
+            }
         }
     }
     if (lane < nb) bc[(size_t)lane * rs] = x;

@@ -61,6 +61,24 @@ def gemm(A, B, C=None, transA=False, transB=False, alpha=1.0, beta=0.0, kscale=N
 MODE_NAMES = {GEMM_F64: "f64", GEMM_F32: "f32", GEMM_F32_ACC64: "f32acc64", GEMM_BF16: "bf16"}
 
 
+def gemm_sym(A, B):
+    """C = A^T B (f64) for a product known to be symmetric: upper tiles only,
+    lower triangle mirrored (rvgp_gemm_sym).  A, B: (K, M), f64 or f32."""
+    K, M = A.shape
+    assert B.shape == (K, M) and A.dtype == B.dtype
+    mode = GEMM_F64 if A.dtype == torch.float64 else GEMM_F32_ACC64
+    C = torch.empty((M, M), dtype=torch.float64, device=A.device)
+    ws_bytes = _lib.lib().rvgp_gemm_ws_bytes(mode, M, M, K, 0)
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=A.device) if ws_bytes > 0 else None
+    ev = instrument.begin()
+    _lib.call("rvgp_gemm_sym", mode, M, K, _lib.ptr(A), _ld(A), _lib.ptr(B), _ld(B), _lib.ptr(C),
+              M, 0, _lib.ptr(ws), ws_bytes, _lib.stream())
+    if ev is not None:
+        instrument.end(ev, "gemm_sym", MODE_NAMES[mode], 2 * M * K * A.element_size() + M * M * 8,
+                       2.0 * M * M * K)
+    return C
+
+
 def potrf(A, check=True):
     """In-place lower Cholesky of a square row-major tensor (f64 or f32)."""
     n = A.shape[0]

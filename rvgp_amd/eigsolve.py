@@ -44,10 +44,9 @@ class EigStats(dict):
 def _rayleigh_ritz(A, Y, work_dtype):
     """Returns X (N x p'), theta (p',) f64, AX (N x p') in work dtype."""
     W = spmm(A, Y)
-    G = dense.gemm(Y, Y, transA=True, out_dtype=torch.float64)
-    H = dense.gemm(Y, W, transA=True, out_dtype=torch.float64)
-    G = 0.5 * (G + G.T)
-    H = 0.5 * (H + H.T)
+    # both Grams are symmetric (A is): upper tiles only, mirrored
+    G = dense.gemm_sym(Y, Y)
+    H = dense.gemm_sym(Y, W)
     drop = 1e-13 if work_dtype == torch.float64 else 1e-10
     Bm = _gram_basis_chol(G, drop)
     if Bm is None:
@@ -162,7 +161,7 @@ def _run_phase(A, X, k, ub, wdt, ptol, max_iter, max_ratio, lock, fresh, stats, 
                   f"worst_res={worst:.3e} spmm={stats['spmm']}", file=sys.stderr, flush=True)
         if kk >= need and worst < ptol:
             X = torch.cat([Xl, Xa], 1)
-            return X, torch.cat([thl, tha]), it
+            return X, torch.cat([thl, tha]), it, True
         # a phase whose residuals have stopped falling (the FP32 rounding
         # floor of the mixed form) hands over to the next phase
         history.append(worst)
@@ -172,7 +171,7 @@ def _run_phase(A, X, k, ub, wdt, ptol, max_iter, max_ratio, lock, fresh, stats, 
                 stats["stalled_" + tag] = worst
             X = torch.cat([Xl, Xa], 1)
             stats["unconverged"] = worst
-            return X, torch.cat([thl, tha]), it
+            return X, torch.cat([thl, tha]), it, False
         if lock:
             conv = (res < ptol).cpu().numpy()
             c = 0
@@ -228,24 +227,26 @@ def chfsi(A: BSR, k, precision="fp64", tol=1e-9, max_iter=300, guard=None, seed=
         # FP32 filter body with locking; hands over at 1e-5, a few times above
         # the FP32 residual floor (a phase that reaches the floor first
         # stalls there)
-        X, theta, it = _run_phase(A, X.to(torch.float32).contiguous(), k, ub, torch.float32, 1e-5,
-                                  max_iter, 300.0, True, fresh, stats, verbose, "fp32",
-                                  stall_exit=True)
+        X, theta, it, _ = _run_phase(A, X.to(torch.float32).contiguous(), k, ub, torch.float32,
+                                     1e-5, max_iter, 300.0, True, fresh, stats, verbose, "fp32",
+                                     stall_exit=True)
         stats["iters_fp32"] = it
         stats["t_fp32"] = time.perf_counter() - t0
         # FP64 polish with locking: the FP32 block is already accurate to
-        # ~1e-6, so one or two filter passes finish most pairs
-        X, theta, it = _run_phase(A, X.double().contiguous(), k, ub, torch.float64, tol, 50, 1e4,
-                                  True, fresh, stats, verbose, "fp64-polish")
-        X, theta, _ = _run_phase(A, X, k, ub, torch.float64, tol, 3, 1e4, False, fresh, stats,
-                                 verbose, "fp64-final", m_max=16)
+        # ~1e-5, so a few filter passes finish the pairs
+        X, theta, it, done = _run_phase(A, X.double().contiguous(), k, ub, torch.float64, tol, 50,
+                                        1e4, True, fresh, stats, verbose, "fp64-polish")
+        if not done:
+            # an unlocked Rayleigh-Ritz over the whole block orders and refines
+            X, theta, _, _ = _run_phase(A, X, k, ub, torch.float64, tol, 3, 1e4, False, fresh,
+                                        stats, verbose, "fp64-final", m_max=16)
         stats["iters_polish"] = it
     elif precision == "fp64":
-        X, theta, it = _run_phase(A, X.to(torch.float64).contiguous(), k, ub, torch.float64, tol,
-                                  max_iter, 1e4, True, fresh, stats, verbose, "fp64")
-        # one unlocked Rayleigh-Ritz over the whole block orders and refines
-        X, theta, _ = _run_phase(A, X, k, ub, torch.float64, tol, 3, 1e4, False, fresh, stats,
-                                 verbose, "fp64-final", m_max=16)
+        X, theta, it, done = _run_phase(A, X.to(torch.float64).contiguous(), k, ub, torch.float64,
+                                        tol, max_iter, 1e4, True, fresh, stats, verbose, "fp64")
+        if not done:
+            X, theta, _, _ = _run_phase(A, X, k, ub, torch.float64, tol, 3, 1e4, False, fresh,
+                                        stats, verbose, "fp64-final", m_max=16)
     else:
         raise ValueError("precision must be 'fp64' or 'mixed'")
     stats["t_total"] = time.perf_counter() - t0

@@ -99,3 +99,24 @@ def test_potrf_not_spd():
     A[40, 40] = -1.0
     with pytest.raises(dense.CholeskyError):
         dense.potrf(_t(A, torch.float64))
+
+
+@pytest.mark.parametrize("K,M", [(40000, 300), (5000, 576), (3000, 90), (257, 130)])
+def test_gemm_sym_matches_full_product(K, M):
+    """Upper-tile symmetric Gram: equals A^T B on the upper triangle and is
+    exactly symmetric; f64 and f32-in/f64-accumulate, with and without split-K."""
+    import torch
+    from rvgp_amd import dense
+    rng = np.random.default_rng(K + M)
+    Y = rng.normal(size=(K, M))
+    W = Y + 0.1 * rng.normal(size=(K, M))
+    for A, B in ((Y, Y), (Y, W)):
+        ref = A.T @ B
+        for dt, tol in ((torch.float64, 1e-11), (torch.float32, 1e-10)):
+            if dt == torch.float32:
+                A32, B32 = A.astype(np.float32).astype(np.float64), B.astype(np.float32).astype(np.float64)
+                ref = A32.T @ B32
+            C = dense.gemm_sym(_t(A, dt), _t(B, dt)).cpu().numpy()
+            iu = np.triu_indices(M)
+            np.testing.assert_allclose(C[iu], ref[iu], rtol=0, atol=tol * np.abs(ref).max())
+            assert np.array_equal(C, C.T)

@@ -29,7 +29,9 @@ constexpr int NB = 64;
 
 // Diagonal block, one wavefront: lane r holds row r of the block in
 // registers; the column loop is fully unrolled, so every register index is a
-// constant and the entries other lanes need arrive by v_readlane.  The chain
+// constant.  The entries of the scaled column that other lanes need are read
+// from LDS as broadcasts (two per 16-byte read), except the look-ahead entry,
+// which arrives by v_readlane.  The chain
 // that bounds the kernel is pivot -> 1/sqrt -> scaled column -> next pivot,
 // so column c+1 is updated first and its pivot's reciprocal square root is
 // in flight while the rest of the rank-1 update issues.  Entries right of a
@@ -40,6 +42,7 @@ constexpr int NB = 64;
 template <typename T>
 __global__ __launch_bounds__(64) void potrf_diag(T* __restrict__ A, int lda, int j0, int nb,
                                                  int* __restrict__ status) {
+    __shared__ T col[NB];
     const int r = threadIdx.x;
     T a[NB];
 #pragma unroll
@@ -53,10 +56,15 @@ __global__ __launch_bounds__(64) void potrf_diag(T* __restrict__ A, int lda, int
         const T inv = rsqrt(piv);
         a[c] = r == c ? piv * inv : (r > c ? a[c] * inv : T(0));
         if (c + 1 < NB) {
+            // publish the scaled column; the workgroup is one wavefront, whose
+            // LDS operations complete in program order, so the reads below see
+            // this write and the next column's write follows these reads
+            col[r] = a[c];
             a[c + 1] -= a[c] * bcast(a[c], c + 1);
             piv = bcast(a[c + 1], c + 1);
 #pragma unroll
-            for (int s = c + 2; s < NB; ++s) a[s] -= a[c] * bcast(a[c], s);
+            for (int s = c + 2; s < NB; ++s) a[s] -= a[c] * col[s];
+            __builtin_amdgcn_wave_barrier();
         }
     }
     if (bad) {  // wave-uniform: every lane saw the same pivots

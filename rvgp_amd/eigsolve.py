@@ -81,6 +81,16 @@ def _gram_basis_chol(G, drop):
     return Linv.T.contiguous()
 
 
+def _cholqr(Y):
+    """Y R^-1 with R^T R = Y^T Y (Cholesky QR; FP64 Gram and factor, result in
+    Y's dtype), or None when the Gram is not numerically positive definite."""
+    G = dense.gemm_sym(Y, Y)
+    B = _gram_basis_chol(G, 1e-13 if Y.dtype == torch.float64 else 1e-10)
+    if B is None:
+        return None
+    return dense.gemm(Y, B.to(Y.dtype).contiguous())
+
+
 def _residuals(X, theta, AX):
     """|A x_j - theta_j x_j| / |x_j| per column (column sums in FP64)."""
     R = AX - X * theta.to(X.dtype)
@@ -119,9 +129,11 @@ def _degree(theta, k, cut, ub, max_ratio, m_min=4, m_max=60, need=None):
     a0 = math.acosh(max(x0, 1.0 + 1e-15))
     ak = math.acosh(max(xk, 1.0 + 1e-15))
     m = m_max if a0 - ak <= 1e-12 else int(math.log(max_ratio) / (a0 - ak))
+    m_need = None
     if need is not None and need > 1.0 and ak > 1e-12:
-        m = min(m, int(math.ceil(math.log(need) / ak)))
-    return max(m_min, min(m_max, m))
+        m_need = int(math.ceil(math.log(need) / ak))
+        m = min(m, m_need)
+    return max(m_min, min(m_max, m)), m_need
 
 
 def _deflate(Y, Xl):
@@ -189,13 +201,30 @@ def _run_phase(A, X, k, ub, wdt, ptol, max_iter, max_ratio, lock, fresh, stats, 
         cut = float(tha[min(tha.numel(), Xa.shape[1]) - 1].item())
         low = float(tha[0].item())
         kk = min(k - Xl.shape[1], tha.numel())
-        m = _degree(tha, max(kk, 1), cut, ub, max_ratio, m_max=m_max,
-                    need=10.0 * worst / ptol if kk > 0 else None)
+        m, m_need = _degree(tha, max(kk, 1), cut, ub, max_ratio, m_max=m_max,
+                            need=10.0 * worst / ptol if kk > 0 else None)
         tf = time.perf_counter()
         Y = _cheb_filter(A, Xa, m, cut, ub, low)
-        torch.cuda.synchronize()
-        stats["t_filter"] += time.perf_counter() - tf
         stats["spmm"] += m
+        if (A.b == 1 and wdt == torch.float32 and Xl.shape[1] == 0 and m_need is not None
+                and m_need > m):
+            # FP32 phase of a scalar operator, degree capped by the block's
+            # conditioning rather than by the residual target:
+            # re-orthonormalise (Cholesky QR; no locked vectors to deflate
+            # yet) and filter again with the same bounds, in place of a full
+            # Rayleigh-Ritz between the two passes.  Only where a block SpMM
+            # is cheap against a Rayleigh-Ritz pass (b = 1: ~48 SpMMs per RR
+            # at p = 576 on MI355X; b = 3: ~18): elsewhere the second
+            # full-width pass costs more than the RR it saves, because it
+            # also delays locking.
+            Q = _cholqr(Y)
+            if Q is not None:
+                m2 = max(4, min(m, m_need - m))
+                Y = _cheb_filter(A, Q, m2, cut, ub, low)
+                stats["spmm"] += m2
+                stats["cholqr"] = stats.get("cholqr", 0) + 1
+        torch.cuda.current_stream().synchronize()  # this solve's stream only
+        stats["t_filter"] += time.perf_counter() - tf
         stats["iters"] += 1
         it += 1
 

@@ -82,8 +82,9 @@ def tangent_frames(X, G, d, K):
     return T.cpu().numpy(), S.cpu().numpy()
 
 
-def connections_device(tangents, g, d, want_R=True, lc64=True, lc32=True):
-    """R blocks (nnz, d, d) and the Laplacian values in block-CSR order."""
+def connections_device(tangents, g, d, want_R=True, lc64=True, lc32=True, scalar=True):
+    """R blocks (nnz, d, d) and the Laplacian values in block-CSR order
+    (scalar=False skips the scalar Laplacian's values)."""
     n, D, ldt = tangents.shape
     if D < d:
         raise ValueError("Embedding dimension must be less or equal to the ambient dimension of input data")
@@ -93,8 +94,8 @@ def connections_device(tangents, g, d, want_R=True, lc64=True, lc32=True):
         "R": torch.empty((nnz, d, d), dtype=torch.float64, device=dev) if want_R else None,
         "lc64": torch.empty((nnz, d, d), dtype=torch.float64, device=dev) if lc64 else None,
         "lc32": torch.empty((nnz, d, d), dtype=torch.float32, device=dev) if lc32 else None,
-        "l64": torch.empty(nnz, dtype=torch.float64, device=dev) if lc64 else None,
-        "l32": torch.empty(nnz, dtype=torch.float32, device=dev) if lc32 else None,
+        "l64": torch.empty(nnz, dtype=torch.float64, device=dev) if (lc64 and scalar) else None,
+        "l32": torch.empty(nnz, dtype=torch.float32, device=dev) if (lc32 and scalar) else None,
     }
     _lib.call("rvgp_connections", _lib.ptr(tangents.contiguous()), n, D, ldt, d,
               _lib.ptr(g.rowptr), _lib.ptr(g.cols), nnz, _lib.ptr(out["R"]), _lib.ptr(out["lc64"]),

@@ -88,6 +88,18 @@ def spmm(A, X, Y=None, alpha=1.0, beta=0.0, Z=None, gamma=0.0, acc=None, delta=0
     return Y
 
 
+def graph_laplacian(G):
+    """L = D - A of the symmetrised graph as a b = 1 BSR on the device
+    (geometry.py:55-63): degree (row length minus the self loop) on the
+    diagonal, -1 off it, in the graph's CSR order -- the same values the
+    connections kernel writes."""
+    rows = torch.repeat_interleave(torch.arange(G.n, device=G.rowptr.device),
+                                   torch.diff(G.rowptr.long()))
+    deg = (torch.diff(G.rowptr) - 1).to(torch.float64)
+    l64 = torch.where(G.cols.long() == rows, deg[rows], torch.full_like(deg[rows], -1.0))
+    return BSR(G.rowptr, G.cols, l64, l64.to(torch.float32), 1)
+
+
 def _vec_ok(r, *ts):
     """Mirror of the kernel's test for the 16-byte vector form: r, every row
     stride and every base address multiples of 16 bytes."""

@@ -210,6 +210,10 @@ def test_connections_vs_reference(case):
     assert abs(B.to_scipy() - Lc).max() < 1e-14
     L = cpu.laplacian(rp, c)
     assert np.abs(out["l64"].cpu().numpy() - L[np.repeat(np.arange(len(X)), np.diff(rp)), c].A1).max() == 0
+    # the graph-only scalar Laplacian (built before the connections) is the same
+    from rvgp_amd.sparse import graph_laplacian
+    Lg = graph_laplacian(g)
+    assert torch.equal(Lg.vals64, out["l64"]) and torch.equal(Lg.vals32, out["l32"])
 
 
 # ---------------------------------------------------------------- SpMM

@@ -70,7 +70,8 @@ def run_fit(X, V, train, precision, args, timings):
     t0 = time.perf_counter()
     d = RVGP.create_data_object(X, vectors=V, n_neighbors=args.k_nn, n_eigenpairs=args.k_eig,
                                 precision=precision, reference_semantics=False, verbose=False,
-                                timings=timings, eig_verbose=args.verbose)
+                                timings=timings, eig_verbose=args.verbose,
+                                eig_tol=args.eig_tol_mixed if precision == "mixed" else None)
     t1 = time.perf_counter()
     gp = RVGP.fit(d, train_ind=train, epochs=args.adam_steps, optimizer="adam", lr=0.05,
                   precision=precision, verbose=False, timings=timings)
@@ -223,6 +224,9 @@ def main():
     p.add_argument("--skip-fp64", action="store_true")
     p.add_argument("--skip-cpu", action="store_true")
     p.add_argument("--cpu-n", dest="cpu_n", type=int, default=1000)
+    p.add_argument("--eig-tol-mixed", dest="eig_tol_mixed", type=float, default=None,
+                   help="eigensolver residual tolerance of the mixed fit (default: the data "
+                        "object's, 1e-7; the FP64 fit uses 1e-9)")
     p.add_argument("--cpu-k-eig", dest="cpu_k_eig", type=int, default=50)
     p.add_argument("--verbose", action="store_true")
     p.add_argument("--roofline-only", dest="roofline_only", action="store_true")

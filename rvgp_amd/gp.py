@@ -7,7 +7,9 @@ hand-written adjoint here) and predict_f.
 Two exact forms of the same model y ~ N(0, Phi diag(S) Phi^T + s2 I):
   * function space: M x M Gram on the matrix cores (rvgp_gemm with the
     spectral filter as k-scale), blocked Cholesky, triangular solves;
-    optional FP32 Cholesky with FP64 iterative refinement of alpha.
+    in mixed precision alpha comes from the FP64 k x k system below
+    (Woodbury), refined against the matrix-free FP64 K_y, and no M x M
+    matrix is formed.
   * weight space (Woodbury / determinant lemma): k x k system
     A = s2 S^-1 + Phi^T Phi; G = Phi^T Phi and b = Phi^T y are formed once per
     training set, so an optimiser step costs O(k^3) and never touches an
@@ -38,8 +40,7 @@ class GPREngine:
     """Hyperparameters theta = (nu, kappa, sigma_f, s2) = softplus(u) + lower."""
 
     def __init__(self, Phi, y, evals, n_rows, typ="matern", form="auto", precision="fp64",
-                 lower=(0.0, 0.0, 0.0, 1e-6), fs_max=8192, gram_mode=None,
-                 fs_gram_mode=dense.GEMM_F32):
+                 lower=(0.0, 0.0, 0.0, 1e-6), fs_max=8192, gram_mode=None):
         dev = Phi.device
         self.dev = dev
         self.M, self.k = Phi.shape
@@ -64,8 +65,8 @@ class GPREngine:
             _lib.lib().rvgp_colsumsq_ws_bytes(max(self.M, self.k), max(self.k, 1)) // 8 + 1, **f64)
         if self.form == "weight" or precision == "mixed":
             # k x k normal-equation system (weight space; in the mixed
-            # function-space form it carries log det and diag(Phi^T K^-1 Phi)
-            # in FP64 so that only the alpha solve uses the FP32 factor)
+            # function-space form it carries log det, diag(Phi^T K^-1 Phi) and
+            # the Woodbury solve for alpha, all in FP64)
             if gram_mode is None:
                 # mixed: FP32 Phi tiles, products accumulated in FP64 on the
                 # matrix cores (an FP32-accumulated G over M = 768k rows costs
@@ -85,7 +86,6 @@ class GPREngine:
             self.Linv = torch.empty((self.k, self.k), **f64)
             self.eye = torch.eye(self.k, **f64)
             self.ainv = torch.empty(self.k, **f64)
-        self.fs_gram_mode = fs_gram_mode
         if self.form == "function":
             if precision != "mixed":
                 self.K = torch.empty((self.M, self.M), **f64)
@@ -139,15 +139,20 @@ class GPREngine:
                       _lib.ptr(self.yy), _lib.ptr(self.S), _lib.ptr(self.dS), _lib.ptr(self.s2),
                       _lib.ptr(self.u), _lib.ptr(self.gS), _lib.ptr(self.out), s)
         else:
-            self._factor()
-            # a = K_y^-1 y: L^-1 y first (ya = |L^-1 y|^2), then L^-T
-            self.a.copy_(self.y)
-            self._solve_L(self.a)
-            ya = dense.gemm(self.a, self.a, transA=True).reshape(1)
-            self._solve_LT(self.a)
             if self.precision == "mixed":
-                self._refine(self.a)
+                # alpha = K_y^-1 y = (y - Phi w) / s2 (Woodbury; w = A^-1 Phi^T y
+                # above), then two refinement sweeps against the FP64 K_y
+                torch.div(self.y - dense.gemm(self.Phi, self.w), self.s2, out=self.a)
+                for _ in range(2):
+                    self.a += self._woodbury(self.y - self._apply_Ky(self.a))
                 ya = dense.gemm(self.y, self.a, transA=True).reshape(1)
+            else:
+                self._factor()
+                # a = K_y^-1 y: L^-1 y first (ya = |L^-1 y|^2), then L^-T
+                self.a.copy_(self.y)
+                dense.trsm(self.K, self.a)
+                ya = dense.gemm(self.a, self.a, transA=True).reshape(1)
+                dense.trsm(self.K, self.a, trans=True)
             aa = dense.gemm(self.a, self.a, transA=True).reshape(1)
             if self.precision == "mixed":
                 _lib.call("rvgp_lemma_terms", self.k, float(self.M), _lib.ptr(self.S),
@@ -155,7 +160,7 @@ class GPREngine:
                           _lib.ptr(self.q), _lib.ptr(self.logdet), s)
             else:
                 self.Z.copy_(self.Phi)
-                self._solve_L(self.Z)
+                dense.trsm(self.K, self.Z)
                 self._colsumsq(self.Z, self.q)
             p = dense.gemm(self.Phi, self.a, transA=True).reshape(-1)
             _lib.call("rvgp_fs_finish", self.k, float(self.M), _lib.ptr(self.q), _lib.ptr(p),
@@ -166,44 +171,13 @@ class GPREngine:
 
     # function-space helpers -------------------------------------------------
     def _factor(self):
-        """K_y = Phi diag(S) Phi^T + s2 I and its Cholesky factor (f64, or f32
-        in mixed precision)."""
-        if self.precision == "mixed":
-            # FP32 (or BF16-tile) Gram on the matrix cores and FP32 factor: the
-            # preconditioner of the FP64-refined alpha solve only (log det comes
-            # from the FP64 determinant-lemma terms)
-            if not hasattr(self, "K32"):
-                self.Phi32 = self.Phi.float().contiguous()
-                self.K32 = torch.empty((self.M, self.M), dtype=torch.float32, device=self.dev)
-            self.K32.zero_()
-            torch.diagonal(self.K32).copy_(self.s2.float().expand(self.M))
-            dense.gemm(self.Phi32, self.Phi32, self.K32, transB=True, alpha=1.0, beta=1.0,
-                       kscale=self.S, mode=self.fs_gram_mode)
-            _lib.call("rvgp_potrf", 0, self.M, _lib.ptr(self.K32), self.M, _lib.ptr(self.status),
-                      _lib.stream())
-        else:
-            self.K.zero_()
-            torch.diagonal(self.K).copy_(self.s2.expand(self.M))
-            dense.gemm(self.Phi, self.Phi, self.K, transB=True, alpha=1.0, beta=1.0, kscale=self.S)
-            _lib.call("rvgp_potrf", 1, self.M, _lib.ptr(self.K), self.M, _lib.ptr(self.status),
-                      _lib.stream())
-            self.logdet.copy_(dense.chol_logdet_half(self.K))
-
-    def _solve_L(self, B):
-        if self.precision == "mixed":
-            B32 = B.float()
-            dense.trsm(self.K32, B32)
-            B.copy_(B32)
-        else:
-            dense.trsm(self.K, B)
-
-    def _solve_LT(self, B):
-        if self.precision == "mixed":
-            B32 = B.float()
-            dense.trsm(self.K32, B32, trans=True)
-            B.copy_(B32)
-        else:
-            dense.trsm(self.K, B, trans=True)
+        """K_y = Phi diag(S) Phi^T + s2 I and its FP64 Cholesky factor."""
+        self.K.zero_()
+        torch.diagonal(self.K).copy_(self.s2.expand(self.M))
+        dense.gemm(self.Phi, self.Phi, self.K, transB=True, alpha=1.0, beta=1.0, kscale=self.S)
+        _lib.call("rvgp_potrf", 1, self.M, _lib.ptr(self.K), self.M, _lib.ptr(self.status),
+                  _lib.stream())
+        self.logdet.copy_(dense.chol_logdet_half(self.K))
 
     def _apply_Ky(self, x):
         """K_y x = Phi (S * (Phi^T x)) + s2 x, matrix-free in FP64."""
@@ -212,15 +186,13 @@ class GPREngine:
         out = dense.gemm(self.Phi, t.contiguous())
         return out + self.s2 * x
 
-    def _refine(self, x, iters=8):
-        """FP64 iterative refinement of K_y x = y with the FP32 factor.  Each
-        sweep contracts the error by ~eps32 * cond(K_y); a fixed count keeps the
-        step free of host synchronisation (graph-capturable)."""
-        for _ in range(iters):
-            r = self.y - self._apply_Ky(x)
-            self._solve_L(r)
-            self._solve_LT(r)
-            x += r
+    def _woodbury(self, r):
+        """K_y^-1 r = (r - Phi A^-1 Phi^T r) / s2, A^-1 = Linv^T Linv: two
+        skinny products with Phi and two k x k products, never an M x M
+        matrix."""
+        t = dense.gemm(self.Linv, dense.gemm(self.Phi, r, transA=True))
+        u = dense.gemm(self.Linv, t, transA=True)
+        return (r - dense.gemm(self.Phi, u)) / self.s2
 
     # ---------------------------------------------------------------- host API
     def nlml_grad_u(self, u):
@@ -252,7 +224,7 @@ class GPREngine:
             p = dense.gemm(self.Phi, self.a, transA=True)
             mean = dense.gemm(Ps, (p * self.S.reshape(-1, 1)).contiguous()).reshape(-1)
             Kms = dense.gemm(self.Phi, Ps, transB=True, kscale=self.S)  # (M, m*)
-            self._solve_L(Kms)
+            dense.trsm(self.K, Kms)
             cs_ws = torch.empty(_lib.lib().rvgp_colsumsq_ws_bytes(max(self.M, self.k), m) // 8 + 1,
                                 dtype=torch.float64, device=self.dev)
             v = torch.empty(m, dtype=torch.float64, device=self.dev)

@@ -115,7 +115,8 @@ def test_predict_vs_oracle():
 
 
 def test_mixed_precision_function_space_within_1e4():
-    """FP32 Cholesky + FP64 iterative refinement vs the FP64 form."""
+    """Mixed function-space form (FP32-tile Gram accumulated in FP64, Woodbury
+    alpha refined against the FP64 K_y) vs the FP64 form."""
     import torch
     Phi, y, lam = problem(M_nodes=500, k=50, seed=11)
     a = engine(Phi, y, lam, 3000.0, "function")

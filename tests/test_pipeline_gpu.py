@@ -83,3 +83,27 @@ def test_fit_reference_quirks(pair):
         RVGP.fit(d, train_ind=np.arange(100), kernel_variance=1.0, epochs=1, verbose=False)
     with pytest.raises(NotImplementedError):
         RVGP.fit(d, train_ind=np.arange(100), n_inducing_points=10, epochs=1, verbose=False)
+
+
+def test_mixed_precision_pipeline_within_1e4(pair):
+    """The mixed form (FP32 eigensolver body, default residual tolerance
+    1e-7 of the spectral bound, FP64 k x k Woodbury system) against
+    the FP64 form on the same points, field and training split."""
+    import rvgp_amd as RVGP
+    _, _, X = pair
+    out = {}
+    for prec in ("fp64", "mixed"):
+        d = RVGP.create_data_object(X, n_neighbors=10, n_eigenpairs=30, precision=prec,
+                                    verbose=False)
+        d.random_vector_field(seed=3)
+        d.smooth_vector_field(t=10)
+        train_ind = np.random.default_rng(5).choice(np.arange(800), size=400, replace=False)
+        gp = RVGP.fit(d, train_ind=train_ind, epochs=30, precision=prec, verbose=False)
+        test_ind = [int(i) for i in range(800) if i not in set(train_ind.tolist())]
+        mean, _ = gp.transform(d, test_ind)
+        out[prec] = (np.asarray(d.evals_Lc), np.asarray(d.vectors), np.asarray(mean))
+    (e64, v64, m64), (em, vm, mm) = out["fp64"], out["mixed"]
+    # |theta - lambda| <= residual <= 1e-7 x the Gershgorin bound (< 4 * 2 * 20 here)
+    np.testing.assert_allclose(em, e64, rtol=0, atol=1e-5)
+    assert np.abs(vm - v64).max() < 1e-4 * np.abs(v64).max()
+    assert np.linalg.norm(mm - m64) / np.linalg.norm(m64) < 1e-4

@@ -147,16 +147,33 @@ def _deflate(Y, Xl):
 
 
 def _run_phase(A, X, k, ub, wdt, ptol, max_iter, max_ratio, lock, fresh, stats, verbose, tag,
-               stall_exit=False, m_max=60):
+               stall_exit=False, m_max=60, ritz=None):
     """Filter + Rayleigh-Ritz iterations on block X (work dtype wdt) until the
     k wanted pairs have residual/ub < ptol.  With ``lock``, converged leading
-    pairs are frozen and deflated out of the filtered block."""
+    pairs are frozen and deflated out of the filtered block.  ``ritz`` =
+    (theta, worst residual / ub) of X when X already is a Ritz basis (the
+    previous phase's result): the phase then opens with a filter pass instead
+    of a Rayleigh-Ritz step, which would only re-derive those values."""
     p = X.shape[1]
     history = []
     Xl = X[:, :0]
     thl = torch.zeros(0, dtype=torch.float64, device=X.device)
     Y = X
     it = 0
+    if ritz is not None:
+        th0, worst0 = ritz
+        tf = time.perf_counter()
+        cut, low = float(th0.max().item()), float(th0.min().item())
+        m, m_need = _degree(torch.sort(th0).values, min(k, th0.numel()), cut, ub, max_ratio,
+                            m_max=m_max, need=10.0 * worst0 / ptol)
+        if verbose:
+            print(f"[chfsi {tag}] opening filter m={m} m_need={m_need} worst={worst0:.3e}",
+                  file=sys.stderr, flush=True)
+        Y = _cheb_filter(A, X, m, cut, ub, low)
+        stats["spmm"] += m
+        torch.cuda.current_stream().synchronize()
+        stats["t_filter"] += time.perf_counter() - tf
+        stats["iters"] += 1
     while True:
         tr = time.perf_counter()
         Y = _deflate(Y, Xl)
@@ -171,6 +188,7 @@ def _run_phase(A, X, k, ub, wdt, ptol, max_iter, max_ratio, lock, fresh, stats, 
         if verbose:
             print(f"[chfsi {tag}] it={it} locked={Xl.shape[1]} active={Xa.shape[1]} "
                   f"worst_res={worst:.3e} spmm={stats['spmm']}", file=sys.stderr, flush=True)
+        stats["last_worst"] = worst
         if kk >= need and worst < ptol:
             X = torch.cat([Xl, Xa], 1)
             return X, torch.cat([thl, tha]), it, True
@@ -203,6 +221,9 @@ def _run_phase(A, X, k, ub, wdt, ptol, max_iter, max_ratio, lock, fresh, stats, 
         kk = min(k - Xl.shape[1], tha.numel())
         m, m_need = _degree(tha, max(kk, 1), cut, ub, max_ratio, m_max=m_max,
                             need=10.0 * worst / ptol if kk > 0 else None)
+        if verbose:
+            print(f"[chfsi {tag}] filter m={m} m_need={m_need} cut={cut:.4e} low={low:.4e}",
+                  file=sys.stderr, flush=True)
         tf = time.perf_counter()
         Y = _cheb_filter(A, Xa, m, cut, ub, low)
         stats["spmm"] += m
@@ -264,7 +285,8 @@ def chfsi(A: BSR, k, precision="fp64", tol=1e-9, max_iter=300, guard=None, seed=
         # FP64 polish with locking: the FP32 block is already accurate to
         # ~1e-5, so a few filter passes finish the pairs
         X, theta, it, done = _run_phase(A, X.double().contiguous(), k, ub, torch.float64, tol, 50,
-                                        1e4, True, fresh, stats, verbose, "fp64-polish")
+                                        1e4, True, fresh, stats, verbose, "fp64-polish",
+                                        ritz=(theta, stats["last_worst"]))
         if not done:
             # an unlocked Rayleigh-Ritz over the whole block orders and refines
             X, theta, _, _ = _run_phase(A, X, k, ub, torch.float64, tol, 3, 1e4, False, fresh,

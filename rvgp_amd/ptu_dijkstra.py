@@ -27,8 +27,14 @@ def _check_status(st):
         raise RuntimeError(f"device status {v}")
 
 
-def geo_nbh_device(g, K, max_ws_bytes=2 << 30):
-    """(n, K+1) int32 neighbourhoods in heap pop order (pyx:361-394)."""
+def geo_nbh_device(g, K, max_ws_bytes=16 << 30):
+    """(n, K+1) int32 neighbourhoods in heap pop order (pyx:361-394).
+
+    One lane per source and every source in flight at once where the
+    workspace cap allows (80 KB per lane at K = 30 and 39-entry rows: 8 GB
+    at n = 1e5): the heap walks are latency-bound, so the number of lanes in
+    flight sets the speed (2 GiB, a quarter of the sources per pass, took
+    twice as long)."""
     n = g.n
     pool = 1 + K * g.max_row()
     per_lane = _lib.lib().rvgp_geo_nbh_ws_bytes(1, pool)

@@ -447,10 +447,11 @@ __global__ __launch_bounds__(256) void gemm_small_kernel(
 //     y = alpha op(A) x + beta y.
 // op = N: one wavefront per row, lanes striding K (coalesced), wave sum.
 // op = T: a workgroup per 64 outputs, lane = output column (coalesced along
-// the row), the four waves splitting K and reducing through LDS in a fixed
-// order.  Accumulates in Acc (f64, or f32 for Mode::F32).
+// the row), sixteen waves splitting K (eight loads in flight per lane) and
+// reducing through LDS in a fixed order.  Accumulates in Acc (f64, or f32
+// for Mode::F32).
 template <Mode MD>
-__global__ __launch_bounds__(256) void gemv_kernel(int transA, int M, int K,
+__global__ __launch_bounds__(1024) void gemv_kernel(int transA, int M, int K,
                                                    typename Tr<MD>::Acc alpha,
                                                    const typename Tr<MD>::In* __restrict__ A,
                                                    int lda, const typename Tr<MD>::In* __restrict__ x,
@@ -472,19 +473,30 @@ __global__ __launch_bounds__(256) void gemv_kernel(int transA, int M, int K,
             y[(size_t)m * incy] = o;
         }
     } else {
-        __shared__ Acc part[4][64];
+        constexpr int NW = 16;
+        __shared__ Acc part[NW][64];
         const int m = blockIdx.x * 64 + lane;
-        const int per = (K + 3) / 4, k0 = w * per, k1 = min(K, k0 + per);
-        Acc p[4] = {Acc(0), Acc(0), Acc(0), Acc(0)};
+        const int per = (K + NW - 1) / NW, k0 = w * per, k1 = min(K, k0 + per);
+        Acc p[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) p[t] = Acc(0);
         if (m < M) {
-            int t = 0;
-            for (int k = k0; k < k1; ++k, t = (t + 1) & 3)
+            int k = k0;
+            for (; k + 7 < k1; k += 8) {
+#pragma unroll
+                for (int t = 0; t < 8; ++t)
+                    p[t] += (Acc)A[(size_t)(k + t) * lda + m] * (Acc)x[(size_t)(k + t) * incx];
+            }
+            for (int t = 0; k < k1; ++k, ++t)
                 p[t] += (Acc)A[(size_t)k * lda + m] * (Acc)x[(size_t)k * incx];
         }
-        part[w][lane] = (p[0] + p[1]) + (p[2] + p[3]);
+        part[w][lane] = ((p[0] + p[1]) + (p[2] + p[3])) + ((p[4] + p[5]) + (p[6] + p[7]));
         __syncthreads();
         if (w == 0 && m < M) {
-            Acc o = alpha * ((part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]));
+            Acc o = Acc(0);
+#pragma unroll
+            for (int q = 0; q < NW; ++q) o += part[q][lane];
+            o *= alpha;
             if (beta != Acc(0)) o += beta * y[(size_t)m * incy];
             y[(size_t)m * incy] = o;
         }
@@ -558,7 +570,7 @@ static int gemm_launch(int transA, int transB, int M, int N, int K, double alpha
         if (N == 1 && K <= 8192 && kscale == nullptr && !lower && !upper && split <= 1) {
             const int incx = transB ? 1 : ldb;
             if (transA)
-                hipLaunchKernelGGL(gemv_kernel<MD>, dim3(cdiv(M, 64)), dim3(256), 0, s, 1, M, K,
+                hipLaunchKernelGGL(gemv_kernel<MD>, dim3(cdiv(M, 64)), dim3(1024), 0, s, 1, M, K,
                                    (Acc)alpha, static_cast<const In*>(A), lda,
                                    static_cast<const In*>(B), incx, (Acc)beta,
                                    static_cast<Acc*>(C), ldc);

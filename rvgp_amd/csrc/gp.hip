@@ -112,7 +112,9 @@ __global__ void ws_form_kernel(int k, const double* __restrict__ G, int ldg,
 // accumulators per lane -- and a second pass sums them in slice order.
 constexpr int kColMaxParts = 1024;
 static inline int colsumsq_parts(int M) {
-    const int p = (M + 255) / 256;
+    // 32-row slices: every thread's loads of a slice are in flight together,
+    // so a column sum costs one memory latency, not M/4 of them
+    const int p = (M + 31) / 32;
     return p < 1 ? 1 : (p > kColMaxParts ? kColMaxParts : p);
 }
 
@@ -123,29 +125,35 @@ __global__ void colsumsq_part_kernel(int M, int k, int parts, const T* __restric
     if (j >= k) return;
     const int per = (M + parts - 1) / parts;
     const int i0 = blockIdx.y * per, i1 = min(M, i0 + per);
-    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    double s[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     int i = i0;
-    for (; i + 3 < i1; i += 4) {
+    for (; i + 7 < i1; i += 8) {
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
+        for (int t = 0; t < 8; ++t) {
             const double z = (double)Z[(size_t)(i + t) * ldz + j];
             s[t] += z * z;
         }
     }
-    for (; i < i1; ++i) {
+    for (int t = 0; i < i1; ++i, ++t) {
         const double z = (double)Z[(size_t)i * ldz + j];
-        s[0] += z * z;
+        s[t] += z * z;
     }
-    part[(size_t)blockIdx.y * k + j] = (s[0] + s[1]) + (s[2] + s[3]);
+    part[(size_t)blockIdx.y * k + j] =
+        ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
 }
 
 __global__ void colsumsq_reduce_kernel(int k, int parts, const double* __restrict__ part,
                                        double* __restrict__ out) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= k) return;
-    double s = 0.0;
-    for (int p = 0; p < parts; ++p) s += part[(size_t)p * k + j];
-    out[j] = s;
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    int p = 0;
+    for (; p + 3 < parts; p += 4) {
+#pragma unroll
+        for (int t = 0; t < 4; ++t) s[t] += part[(size_t)(p + t) * k + j];
+    }
+    for (int t = 0; p < parts; ++p, ++t) s[t] += part[(size_t)p * k + j];
+    out[j] = (s[0] + s[1]) + (s[2] + s[3]);
 }
 
 // Map dNLML/dS, dNLML/ds2 to theta = (nu, kappa, sigma_f, s2) and to the

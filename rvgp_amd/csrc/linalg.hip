@@ -3,7 +3,7 @@
 // RVGP/main.py:89-91 and 110-111).
 //
 // potrf: right-looking, NB = 64.  Per block column: (1) the diagonal block
-// is factorised by one wavefront with its rows in registers, (2) the panel
+// is factorised by four wavefronts with its rows in registers, (2) the panel
 // below is solved against it (one wavefront per panel row), (3) the lower
 // triangle of the trailing matrix gets A22 -= A21 A21^T on the matrix cores.
 // trsm: blocked right-looking substitution; the off-diagonal updates are
@@ -27,54 +27,75 @@ int gemm_syrk_lower(int f64, int n, int k, double alpha, const void* A, int lda,
 
 constexpr int NB = 64;
 
-// Diagonal block, one wavefront: lane r holds row r of the block in
-// registers; the column loop is fully unrolled, so every register index is a
-// constant.  The entries of the scaled column that other lanes need are read
-// from LDS as broadcasts (two per 16-byte read), except the look-ahead entry,
-// which arrives by v_readlane.  The chain
-// that bounds the kernel is pivot -> 1/sqrt -> scaled column -> next pivot,
-// so column c+1 is updated first and its pivot's reciprocal square root is
-// in flight while the rest of the rank-1 update issues.  Entries right of a
-// lane's diagonal are scratch: a lane clears its entry c when column c is
-// scaled, so nothing it accumulates there reaches the factor, and the update
-// runs on every lane without a per-entry predicate.  Rows past nb are
-// identity rows.
+// Diagonal block, one workgroup of four wavefronts: lane r holds row r of
+// the block and wave w holds columns [16w, 16w+16) of it in registers, every
+// register index a compile-time constant (the panel loop is unrolled).  The
+// block is factored in panels of four columns: the wave owning the panel
+// factors it in registers -- a four-step pivot chain, the entries other
+// lanes need arriving by v_readlane -- and publishes it through LDS (double
+// buffered); after one barrier every wave applies the rank-4 update to its
+// own columns right of the panel (the owner's next panel first).  Per panel
+// the chain is four pivots plus one LDS round trip, instead of one LDS round
+// trip per column.  Entries right of a lane's diagonal are scratch: a lane
+// clears its entry c when column c is scaled, so nothing accumulated there
+// reaches the factor.  Rows past nb are identity rows.
 template <typename T>
-__global__ __launch_bounds__(64) void potrf_diag(T* __restrict__ A, int lda, int j0, int nb,
-                                                 int* __restrict__ status) {
-    __shared__ T col[NB];
-    const int r = threadIdx.x;
-    T a[NB];
+__global__ __launch_bounds__(256) void potrf_diag(T* __restrict__ A, int lda, int j0, int nb,
+                                                  int* __restrict__ status) {
+    __shared__ T P[2][NB][4];
+    __shared__ int bad_any;
+    const int r = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (threadIdx.x == 0) bad_any = 0;
+    T a[16];
 #pragma unroll
-    for (int c = 0; c < NB; ++c)
-        a[c] = (r < nb && c <= r) ? A[(size_t)(j0 + r) * lda + j0 + c] : (c == r ? T(1) : T(0));
-    T piv = bcast(a[0], 0);
+    for (int i = 0; i < 16; ++i) {
+        const int c = 16 * w + i;
+        a[i] = (r < nb && c <= r) ? A[(size_t)(j0 + r) * lda + j0 + c] : (c == r ? T(1) : T(0));
+    }
     bool bad = false;
 #pragma unroll
-    for (int c = 0; c < NB; ++c) {
-        bad |= !(piv > T(0));
-        const T inv = rsqrt(piv);
-        a[c] = r == c ? piv * inv : (r > c ? a[c] * inv : T(0));
-        if (c + 1 < NB) {
-            // publish the scaled column; the workgroup is one wavefront, whose
-            // LDS operations complete in program order, so the reads below see
-            // this write and the next column's write follows these reads
-            col[r] = a[c];
-            a[c + 1] -= a[c] * bcast(a[c], c + 1);
-            piv = bcast(a[c + 1], c + 1);
+    for (int p = 0; p < NB / 4; ++p) {
+        const int lc = (p & 3) * 4;  // the panel's first column in its owner's slice
+        if (w == (p >> 2)) {
 #pragma unroll
-            for (int s = c + 2; s < NB; ++s) a[s] -= a[c] * col[s];
-            __builtin_amdgcn_wave_barrier();
+            for (int j = 0; j < 4; ++j) {
+                const int c = 4 * p + j;
+                const T piv = bcast(a[lc + j], c);
+                bad |= !(piv > T(0));
+                const T inv = rsqrt(piv);
+                a[lc + j] = r == c ? piv * inv : (r > c ? a[lc + j] * inv : T(0));
+#pragma unroll
+                for (int k = j + 1; k < 4; ++k) a[lc + k] -= a[lc + j] * bcast(a[lc + j], 4 * p + k);
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) P[p & 1][r][j] = a[lc + j];
+            if (bad && r == 0) bad_any = 1;
+        }
+        __syncthreads();
+        if (p + 1 < NB / 4) {
+            const T pr0 = P[p & 1][r][0], pr1 = P[p & 1][r][1];
+            const T pr2 = P[p & 1][r][2], pr3 = P[p & 1][r][3];
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                if (16 * w + i > 4 * p + 3) {  // wave-uniform
+                    const T* ps = P[p & 1][16 * w + i];
+                    a[i] -= pr0 * ps[0] + pr1 * ps[1] + pr2 * ps[2] + pr3 * ps[3];
+                }
+            }
         }
     }
-    if (bad) {  // wave-uniform: every lane saw the same pivots
-        if (r == 0) atomicMax(status, RVGP_ST_NOT_SPD);
+    __syncthreads();
+    if (bad_any) {
+        if (threadIdx.x == 0) atomicMax(status, RVGP_ST_NOT_SPD);
         return;
     }
     if (r < nb) {
 #pragma unroll
-        for (int c = 0; c < NB; ++c)
-            if (c <= r) A[(size_t)(j0 + r) * lda + j0 + c] = a[c];
+        for (int i = 0; i < 16; ++i) {
+            const int c = 16 * w + i;
+            if (c <= r) A[(size_t)(j0 + r) * lda + j0 + c] = a[i];
+        }
     }
 }
 
@@ -96,21 +117,33 @@ __global__ __launch_bounds__(256) void trsm_diag(int trans, const T* __restrict_
                                                  int i0, int nb, int r, T* __restrict__ B,
                                                  long long rs, long long cs,
                                                  const int* __restrict__ status) {
-    if (status != nullptr && *status) return;
     __shared__ T L[NB][NB + 1];
     __shared__ T dinv[NB];
     const int tid = threadIdx.x;
-    for (int e = tid; e < NB * NB; e += 256) {  // whole tile: zeros outside the block
-        int a = e / NB, b = e - a * NB;
-        L[a][b] = (a < nb && b <= a) ? Lm[(size_t)(i0 + a) * lda + i0 + b] : T(0);
-    }
-    if (tid < NB) dinv[tid] = tid < nb ? T(1) / Lm[(size_t)(i0 + tid) * lda + i0 + tid] : T(1);
-    __syncthreads();
     const int lane = tid & 63;
     const int c = blockIdx.x * 4 + (tid >> 6);
-    if (c >= r) return;
+    // every global load (status word, tile, this wave's right-hand side) is
+    // issued before the first wait, so they cost one memory latency together
+    const int st = status != nullptr ? *status : 0;
     T* bc = B + (size_t)c * cs;
-    T x = lane < nb ? bc[(size_t)lane * rs] : T(0);
+    const T x0 = c < r && lane < nb ? bc[(size_t)lane * rs] : T(0);
+    T t[NB * NB / 256];
+#pragma unroll
+    for (int q = 0; q < NB * NB / 256; ++q) {  // whole tile: zeros outside the block
+        const int e = tid + 256 * q, a = e / NB, b = e - a * NB;
+        t[q] = (a < nb && b <= a) ? Lm[(size_t)(i0 + a) * lda + i0 + b] : T(0);
+    }
+    const T ld = tid < nb ? Lm[(size_t)(i0 + tid) * lda + i0 + tid] : T(1);
+    if (st) return;  // the same word for the whole grid
+#pragma unroll
+    for (int q = 0; q < NB * NB / 256; ++q) {
+        const int e = tid + 256 * q, a = e / NB, b = e - a * NB;
+        L[a][b] = t[q];
+    }
+    if (tid < NB) dinv[tid] = T(1) / ld;
+    __syncthreads();
+    if (c >= r) return;
+    T x = x0;
     const T dv = dinv[lane];
     T l[NB];
     if (!trans) {
@@ -157,7 +190,7 @@ static int potrf_impl(int n, T* A, int lda, int* status, void* stream) {
     hipStream_t s = as_stream(stream);
     for (int j0 = 0; j0 < n; j0 += NB) {
         const int nb = min(NB, n - j0);
-        hipLaunchKernelGGL(potrf_diag<T>, dim3(1), dim3(64), 0, s, A, lda, j0, nb, status);
+        hipLaunchKernelGGL(potrf_diag<T>, dim3(1), dim3(256), 0, s, A, lda, j0, nb, status);
         RVGP_LAUNCH_CHECK();
         const int rest = n - j0 - nb;
         if (rest <= 0) break;

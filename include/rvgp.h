@@ -240,18 +240,30 @@ int rvgp_spectral_filter(int typ, int k, const double* evals, const double* thet
 int rvgp_ws_form(int k, const double* G, int ldg, const double* S, const double* s2, double* A,
                  int lda, void* stream);
 
-/* Weight-space NLML and gradients.  out (9,) = [nlml, d/d(nu,kappa,sigma_f,s2),
- * d/du(...)]; gS: scratch (k,).  M = number of observations. */
-int rvgp_ws_finish(int k, double M, const double* L, int ldl, const double* ainv_diag,
+/* Weight-space NLML and gradients, summed over nrhs right-hand sides that
+ * share the hyperparameters (GPflow's GPR with a Y of nrhs columns sums the
+ * columns' log-densities).  b = Phi^T Y, w = A^-1 b, Gw = G w are k x nrhs
+ * row-major (leading dimension nrhs); yy = |Y|_F^2.  out (9,) = [nlml,
+ * d/d(nu,kappa,sigma_f,s2), d/du(...)]; gS: scratch (k,).  M = observations
+ * per right-hand side. */
+int rvgp_ws_finish(int k, double M, int nrhs, const double* L, int ldl, const double* ainv_diag,
                    const double* b, const double* w, const double* Gw, const double* yy,
                    const double* S, const double* dS, const double* s2, const double* u, double* gS,
                    double* out, void* stream);
 
-/* Function-space NLML and gradients from q_j = |L^-1 phi_j|^2, p_j = phi_j^T a,
- * ya = y^T a, aa = a^T a and logdet_half = sum log L_ii (all device). */
-int rvgp_fs_finish(int k, double M, const double* q, const double* p, const double* ya,
+/* Function-space NLML and gradients over nrhs right-hand sides from
+ * q_j = |L^-1 phi_j|^2, p = Phi^T a (k x nrhs row-major), ya = sum_r y_r^T a_r,
+ * aa = |a|_F^2 and logdet_half = sum log L_ii (all device). */
+int rvgp_fs_finish(int k, double M, int nrhs, const double* q, const double* p, const double* ya,
                    const double* aa, const double* logdet_half, const double* S, const double* dS,
                    const double* s2, const double* u, double* gS, double* out, void* stream);
+
+/* out[0] = sum_i x_i y_i (f64, fixed summation order); ws of
+ * rvgp_dot_ws_bytes(n) bytes.  The Frobenius products y^T a and |a|^2 of the
+ * multi-right-hand-side NLML. */
+size_t rvgp_dot_ws_bytes(size_t n);
+int rvgp_dot(size_t n, const double* x, const double* y, double* out, void* ws, size_t ws_bytes,
+             void* stream);
 
 /* Determinant-lemma terms for the mixed-precision function-space form, from
  * the FP64 factor LA of A = s2 S^-1 + Phi^T Phi and diag(A^-1):
@@ -268,6 +280,47 @@ int rvgp_gp_params(const double* u, const double* lower, double* kernel_theta, d
 int rvgp_adam_step(double* u, const double* g, double* m, double* v, double* t,
                    const int* trainable, double lr, double b1, double b2, double eps,
                    void* stream);
+
+/* ---------------------------------------------------------------------
+ * Batched fits: F independent fields on one manifold, each with its own
+ * training subset and hyperparameters.  The reference fits one field per
+ * RVGP.fit call (RVGP/main.py:11-84 -> GPflow GPR + scipy optimiser,
+ * main.py:87-95); these entry points run every fit of a step in one launch.
+ * ------------------------------------------------------------------- */
+
+/* G (F, k, k) f64: G_f = sum over the fit's nodes i and p < d of
+ * U[i*d+p]^T U[i*d+p] (the weight-space Gram Phi_f^T Phi_f; the gauges have
+ * orthonormal columns, so the intrinsic rows give the ambient Gram).
+ * U (n*d, k) f64 (f64 = 1) or f32 (widened to f64 on the matrix cores);
+ * the nodes of fit f are nodes[node_off[f] .. node_off[f+1]);
+ * tile = 32, 64 or 128 (the autotuner's choice). */
+int rvgp_batch_gram(int f64, int F, int k, int d, const void* U, int ldu, const int* node_off,
+                    const int* nodes, int tile, double* G, void* stream);
+
+/* b (F, k): b_f = sum_i U_i^T c_i with the intrinsic targets c (one d-row per
+ * listed node, c_i = T_i^T y_i); yy (F): sum of ynorm2 (|y_i|^2 per listed
+ * node) over the fit's nodes. */
+int rvgp_batch_rhs(int F, int k, int d, const double* U, int ldu, const int* node_off,
+                   const int* nodes, const double* c, const double* ynorm2, double* b, double* yy,
+                   void* stream);
+
+/* One optimiser step of all F fits in one persistent launch (a workgroup
+ * per fit, grid = min(grid, F)): theta_f = softplus(u_f) + lower, spectral
+ * filter of the shared evals, A_f = G_f + s2 diag(S)^-1, Cholesky of A_f with
+ * L^-1 formed alongside, the weight-space NLML and adjoint of
+ * rvgp_ws_finish into out (F, 9), and -- when do_adam -- one Adam update of
+ * u_f with state m1, m2 (F, 4) and step counters t (F).  M (F): observations
+ * per fit (ambient rows).  panel = 16 or 32 (Cholesky block width; needs
+ * 8*(panel+1)*(k+panel) bytes of LDS <= 160 KiB).  ws: rvgp_batch_step_ws_bytes
+ * bytes; per fit it holds [L | L^-1] (k x 2k) then S, dS, gS, diag(A^-1),
+ * L^-1 b, w = A^-1 b, G w.  status (F): RVGP_ST_NOT_SPD for a fit whose A
+ * is not positive definite. */
+size_t rvgp_batch_step_ws_bytes(int F, int k);
+int rvgp_batch_step(int F, int k, int typ, const double* evals, double n_rows, const double* G,
+                    const double* b, const double* yy, const double* M, const double* lower,
+                    double* u, const int* trainable, double* m1, double* m2, double* t,
+                    int do_adam, double lr, double b1, double b2, double eps, int panel, int grid,
+                    double* ws, size_t ws_bytes, double* out, int* status, void* stream);
 
 #ifdef __cplusplus
 }

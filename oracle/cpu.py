@@ -398,26 +398,34 @@ def spectral_filter(evals, nu, kappa, sigma_f, n_rows, typ="matern"):
 # a14 / a16  GP regression, textbook (Rasmussen & Williams Alg. 2.1)
 # --------------------------------------------------------------------------
 def gpr_nlml(Phi, y, S, noise):
-    """-log N(y | 0, Phi diag(S) Phi^T + noise I), function space."""
+    """-log N(y | 0, Phi diag(S) Phi^T + noise I), function space.  A 2-D y
+    (M, R) holds R right-hand sides sharing the hyperparameters; their
+    NLMLs are summed (GPflow's GPR.log_marginal_likelihood sums the
+    log-density over the columns of Y)."""
     K = (Phi * S) @ Phi.T
     M = K.shape[0]
+    R = y.shape[1] if np.ndim(y) == 2 else 1
     L = np.linalg.cholesky(K + noise * np.eye(M))
     a = scipy.linalg.solve_triangular(L, y, lower=True)
-    nlml = 0.5 * float((a * a).sum()) + float(np.log(np.diag(L)).sum()) + 0.5 * M * np.log(2 * np.pi)
+    nlml = (0.5 * float((a * a).sum()) + R * float(np.log(np.diag(L)).sum())
+            + 0.5 * R * M * np.log(2 * np.pi))
     return nlml
 
 
 def gpr_grad(Phi, y, S, noise):
-    """(dNLML/dS (k,), dNLML/dnoise) from 1/2 tr((Ky^-1 - a a^T) dKy)."""
+    """(dNLML/dS (k,), dNLML/dnoise) from 1/2 tr((R Ky^-1 - A A^T) dKy),
+    A = Ky^-1 Y over the R right-hand sides."""
     K = (Phi * S) @ Phi.T
     M = K.shape[0]
+    Y = y.reshape(M, -1)
+    R = Y.shape[1]
     Ky = K + noise * np.eye(M)
     Kinv = np.linalg.inv(Ky)
-    a = Kinv @ y.ravel()
+    A = Kinv @ Y
     PtKP = np.einsum("mj,mn,nj->j", Phi, Kinv, Phi)
-    Pta = Phi.T @ a
-    gS = 0.5 * (PtKP - Pta ** 2)
-    gn = 0.5 * (np.trace(Kinv) - a @ a)
+    PtA = Phi.T @ A
+    gS = 0.5 * (R * PtKP - (PtA ** 2).sum(1))
+    gn = 0.5 * (R * np.trace(Kinv) - (A * A).sum())
     return gS, gn
 
 

@@ -23,6 +23,9 @@ def __getattr__(name):
     if name == "fit":
         from .main import train_gp
         return train_gp
+    if name == "fit_batched":
+        from .batch import fit_batched
+        return fit_batched
     if name == "kernels":
         from . import kernels
         return kernels

@@ -49,11 +49,18 @@ SIGNATURES = {
     "rvgp_colsumsq": (I, [I, I, I, P, I, P, P, S, P]),
     "rvgp_spectral_filter": (I, [I, I, P, P, D, P, P, P]),
     "rvgp_ws_form": (I, [I, P, I, P, P, P, I, P]),
-    "rvgp_ws_finish": (I, [I, D, P, I, P, P, P, P, P, P, P, P, P, P, P, P]),
-    "rvgp_fs_finish": (I, [I, D, P, P, P, P, P, P, P, P, P, P, P, P]),
+    "rvgp_ws_finish": (I, [I, D, I, P, I, P, P, P, P, P, P, P, P, P, P, P, P]),
+    "rvgp_fs_finish": (I, [I, D, I, P, P, P, P, P, P, P, P, P, P, P, P]),
+    "rvgp_dot_ws_bytes": (S, [S]),
+    "rvgp_dot": (I, [S, P, P, P, P, S, P]),
     "rvgp_lemma_terms": (I, [I, D, P, P, P, P, I, P, P, P]),
     "rvgp_gp_params": (I, [P, P, P, P, P]),
     "rvgp_adam_step": (I, [P, P, P, P, P, P, D, D, D, D, P]),
+    "rvgp_batch_gram": (I, [I, I, I, I, P, I, P, P, I, P, P]),
+    "rvgp_batch_rhs": (I, [I, I, I, P, I, P, P, P, P, P, P, P]),
+    "rvgp_batch_step_ws_bytes": (S, [I, I]),
+    "rvgp_batch_step": (I, [I, I, I, P, D, P, P, P, P, P, P, P, P, P, P, I, D, D, D, D, I, I, P, S,
+                            P, P, P]),
 }
 
 RVGP_EINVAL, RVGP_ERUNTIME, RVGP_ENOMEM, RVGP_EHIP = -1, -2, -3, -4

@@ -156,6 +156,38 @@ __global__ void colsumsq_reduce_kernel(int k, int parts, const double* __restric
     out[j] = (s[0] + s[1]) + (s[2] + s[3]);
 }
 
+// Dot product out = sum_i x_i y_i in f64, fixed order: P slices of 2048
+// elements (P <= 1024) write partial sums, one workgroup adds them in slice
+// order.  Used for y^T a and |a|^2 over all R right-hand sides at once.
+constexpr int kDotSlice = 2048, kDotMaxParts = 1024;
+static inline int dot_parts(size_t n) {
+    const size_t p = (n + kDotSlice - 1) / kDotSlice;
+    return p < 1 ? 1 : (p > (size_t)kDotMaxParts ? kDotMaxParts : (int)p);
+}
+
+__global__ __launch_bounds__(FT) void dot_part_kernel(size_t n, int parts,
+                                                      const double* __restrict__ x,
+                                                      const double* __restrict__ y,
+                                                      double* __restrict__ part) {
+    __shared__ double sh[FT / 64];
+    const size_t per = (n + parts - 1) / parts;
+    const size_t i0 = (size_t)blockIdx.x * per;
+    const size_t i1 = i0 + per < n ? i0 + per : n;
+    double s = 0.0;
+    for (size_t i = i0 + threadIdx.x; i < i1; i += FT) s += x[i] * y[i];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(FT) void dot_reduce_kernel(int parts, const double* __restrict__ part,
+                                                        double* __restrict__ out) {
+    __shared__ double sh[FT / 64];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < parts; i += FT) s += part[i];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) out[0] = s;
+}
+
 // Map dNLML/dS, dNLML/ds2 to theta = (nu, kappa, sigma_f, s2) and to the
 // unconstrained u (theta = softplus(u) + lower => dtheta/du = sigmoid(u)).
 __device__ void finish_grads(int k, const double* gS, double gs2, const double* dS,
@@ -182,11 +214,15 @@ __device__ void finish_grads(int k, const double* gS, double gs2, const double* 
     }
 }
 
-// Weight-space finish.  Inputs: L = chol(A) (lower), ainv_diag = diag(A^-1),
-// w = A^-1 b, Gw = G w, yy = y^T y.  out: [nlml, dnu, dkappa, dsigma_f, ds2,
-// du_nu, du_kappa, du_sigma_f, du_s2]; scratch gS (k).
-__global__ __launch_bounds__(FT) void ws_finish_kernel(int k, double M, const double* __restrict__ L,
-                                                       int ldl, const double* __restrict__ ainv_diag,
+// Weight-space finish over R right-hand sides sharing the hyperparameters
+// (the NLML is the sum of the R columns' NLMLs, as GPflow's GPR sums the
+// log-density over the columns of Y).  Inputs: L = chol(A) (lower),
+// ainv_diag = diag(A^-1), b = Phi^T Y, w = A^-1 b, Gw = G w (all k x R,
+// row-major, leading dimension R), yy = |Y|_F^2.  out: [nlml, dnu, dkappa,
+// dsigma_f, ds2, du_nu, du_kappa, du_sigma_f, du_s2]; scratch gS (k).
+__global__ __launch_bounds__(FT) void ws_finish_kernel(int k, double M, int R,
+                                                       const double* __restrict__ L, int ldl,
+                                                       const double* __restrict__ ainv_diag,
                                                        const double* __restrict__ b,
                                                        const double* __restrict__ w,
                                                        const double* __restrict__ Gw,
@@ -199,16 +235,22 @@ __global__ __launch_bounds__(FT) void ws_finish_kernel(int k, double M, const do
                                                        double* __restrict__ out) {
     __shared__ double sh[FT / 64];
     const double s2 = s2p[0];
+    const double Rd = (double)R;
     double bw = 0.0, wGw = 0.0, ld = 0.0, lS = 0.0, tr = 0.0;
     for (int j = threadIdx.x; j < k; j += FT) {
-        bw += b[j] * w[j];
-        wGw += w[j] * Gw[j];
+        double ww = 0.0;
+        for (int r = 0; r < R; ++r) {
+            const size_t e = (size_t)j * R + r;
+            bw += b[e] * w[e];
+            wGw += w[e] * Gw[e];
+            ww += w[e] * w[e];
+        }
         ld += log(L[(size_t)j * ldl + j]);
         lS += log(S[j]);
         const double ainv = ainv_diag[j];  // (A^-1)_jj = sum_i (L^-1)_ij^2
         tr += ainv / S[j];
         const double sj = S[j];
-        gS[j] = 0.5 * (1.0 / sj - s2 * ainv / (sj * sj) - w[j] * w[j] / (sj * sj));
+        gS[j] = 0.5 * (Rd / sj - Rd * s2 * ainv / (sj * sj) - ww / (sj * sj));
     }
     bw = block_sum(bw, sh);
     wGw = block_sum(wGw, sh);
@@ -217,18 +259,20 @@ __global__ __launch_bounds__(FT) void ws_finish_kernel(int k, double M, const do
     tr = block_sum(tr, sh);
     const double y2 = yy[0];
     const double two_pi = 6.283185307179586;
-    const double nlml = 0.5 * (y2 - bw) / s2 + 0.5 * ((M - k) * log(s2) + lS + 2.0 * ld) +
-                        0.5 * M * log(two_pi);
+    const double nlml = 0.5 * (y2 - bw) / s2 + 0.5 * Rd * ((M - k) * log(s2) + lS + 2.0 * ld) +
+                        0.5 * Rd * M * log(two_pi);
     const double res2 = y2 - 2.0 * bw + wGw;
-    const double gs2 = 0.5 * ((M - k) / s2 + tr - res2 / (s2 * s2));
+    const double gs2 = 0.5 * (Rd * (M - k) / s2 + Rd * tr - res2 / (s2 * s2));
     if (threadIdx.x == 0) out[0] = nlml;
     __syncthreads();
     finish_grads(k, gS, gs2, dS, u, out, sh);
 }
 
-// Function-space finish.  Inputs: q_j = |L^-1 phi_j|^2, p_j = phi_j^T a,
-// ya = y^T a, aa = a^T a, logdet_half = sum log L_ii.
-__global__ __launch_bounds__(FT) void fs_finish_kernel(int k, double M, const double* __restrict__ q,
+// Function-space finish over R right-hand sides.  Inputs: q_j =
+// |L^-1 phi_j|^2, p = Phi^T a (k x R, row-major, leading dimension R),
+// ya = sum_r y_r^T a_r, aa = |a|_F^2, logdet_half = sum log L_ii.
+__global__ __launch_bounds__(FT) void fs_finish_kernel(int k, double M, int R,
+                                                       const double* __restrict__ q,
                                                        const double* __restrict__ p,
                                                        const double* __restrict__ ya,
                                                        const double* __restrict__ aa,
@@ -241,16 +285,19 @@ __global__ __launch_bounds__(FT) void fs_finish_kernel(int k, double M, const do
                                                        double* __restrict__ out) {
     __shared__ double sh[FT / 64];
     const double s2 = s2p[0];
+    const double Rd = (double)R;
     double sq = 0.0;
     for (int j = threadIdx.x; j < k; j += FT) {
         sq += S[j] * q[j];
-        gS[j] = 0.5 * (q[j] - p[j] * p[j]);
+        double pp = 0.0;
+        for (int r = 0; r < R; ++r) pp += p[(size_t)j * R + r] * p[(size_t)j * R + r];
+        gS[j] = 0.5 * (Rd * q[j] - pp);
     }
     sq = block_sum(sq, sh);
     const double two_pi = 6.283185307179586;
-    const double nlml = 0.5 * ya[0] + ldh[0] + 0.5 * M * log(two_pi);
+    const double nlml = 0.5 * ya[0] + Rd * ldh[0] + 0.5 * Rd * M * log(two_pi);
     const double trinv = (M - sq) / s2;
-    const double gs2 = 0.5 * (trinv - aa[0]);
+    const double gs2 = 0.5 * (Rd * trinv - aa[0]);
     if (threadIdx.x == 0) out[0] = nlml;
     __syncthreads();
     finish_grads(k, gS, gs2, dS, u, out, sh);
@@ -335,25 +382,42 @@ extern "C" int rvgp_colsumsq(int f64, int M, int k, const void* Z, int ldz, doub
     return RVGP_OK;
 }
 
-extern "C" int rvgp_ws_finish(int k, double M, const double* L, int ldl, const double* ainv_diag,
-                              const double* b, const double* w, const double* Gw,
-                              const double* yy, const double* S, const double* dS,
-                              const double* s2, const double* u, double* gS, double* out,
-                              void* stream) {
-    RVGP_CHECK_ARG(k >= 1 && ldl >= k, "rvgp_ws_finish: bad sizes");
-    hipLaunchKernelGGL(ws_finish_kernel, dim3(1), dim3(FT), 0, as_stream(stream), k, M, L, ldl, ainv_diag,
+extern "C" int rvgp_ws_finish(int k, double M, int nrhs, const double* L, int ldl,
+                              const double* ainv_diag, const double* b, const double* w,
+                              const double* Gw, const double* yy, const double* S,
+                              const double* dS, const double* s2, const double* u, double* gS,
+                              double* out, void* stream) {
+    RVGP_CHECK_ARG(k >= 1 && ldl >= k && nrhs >= 1, "rvgp_ws_finish: bad sizes");
+    hipLaunchKernelGGL(ws_finish_kernel, dim3(1), dim3(FT), 0, as_stream(stream), k, M, nrhs, L, ldl,
+                       ainv_diag,
                        b, w, Gw, yy, S, dS, s2, u, gS, out);
     RVGP_LAUNCH_CHECK();
     return RVGP_OK;
 }
 
-extern "C" int rvgp_fs_finish(int k, double M, const double* q, const double* p, const double* ya,
-                              const double* aa, const double* logdet_half, const double* S,
-                              const double* dS, const double* s2, const double* u, double* gS,
-                              double* out, void* stream) {
-    RVGP_CHECK_ARG(k >= 1, "rvgp_fs_finish: k >= 1");
-    hipLaunchKernelGGL(fs_finish_kernel, dim3(1), dim3(FT), 0, as_stream(stream), k, M, q, p, ya, aa,
+extern "C" int rvgp_fs_finish(int k, double M, int nrhs, const double* q, const double* p,
+                              const double* ya, const double* aa, const double* logdet_half,
+                              const double* S, const double* dS, const double* s2, const double* u,
+                              double* gS, double* out, void* stream) {
+    RVGP_CHECK_ARG(k >= 1 && nrhs >= 1, "rvgp_fs_finish: bad sizes");
+    hipLaunchKernelGGL(fs_finish_kernel, dim3(1), dim3(FT), 0, as_stream(stream), k, M, nrhs, q, p, ya,
+                       aa,
                        logdet_half, S, dS, s2, u, gS, out);
+    RVGP_LAUNCH_CHECK();
+    return RVGP_OK;
+}
+
+extern "C" size_t rvgp_dot_ws_bytes(size_t n) { return sizeof(double) * (size_t)dot_parts(n); }
+
+extern "C" int rvgp_dot(size_t n, const double* x, const double* y, double* out, void* ws,
+                        size_t ws_bytes, void* stream) {
+    RVGP_CHECK_ARG(ws_bytes >= rvgp_dot_ws_bytes(n), "rvgp_dot: workspace too small");
+    const int parts = dot_parts(n);
+    double* part = static_cast<double*>(ws);
+    hipLaunchKernelGGL(dot_part_kernel, dim3(parts), dim3(FT), 0, as_stream(stream), n, parts, x, y,
+                       part);
+    RVGP_LAUNCH_CHECK();
+    hipLaunchKernelGGL(dot_reduce_kernel, dim3(1), dim3(FT), 0, as_stream(stream), parts, part, out);
     RVGP_LAUNCH_CHECK();
     return RVGP_OK;
 }

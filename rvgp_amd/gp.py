@@ -49,7 +49,12 @@ class GPREngine:
         self.precision = precision
         self.form = ("function" if self.M <= fs_max else "weight") if form == "auto" else form
         self.Phi = Phi.to(torch.float64).contiguous()
-        self.y = y.to(torch.float64).reshape(-1, 1).contiguous()
+        # y: (M,) or (M, R) -- R right-hand sides sharing the hyperparameters,
+        # NLML summed over them
+        y = y.to(torch.float64)
+        self.y = (y.reshape(-1, 1) if y.dim() == 1 else y).contiguous()
+        assert self.y.shape[0] == self.M, (self.y.shape, self.M)
+        self.R = self.y.shape[1]
         self.evals = evals.to(torch.float64).contiguous()
         f64 = dict(dtype=torch.float64, device=dev)
         self.u = torch.zeros(4, **f64)
@@ -61,6 +66,10 @@ class GPREngine:
         self.gS = torch.empty(self.k, **f64)
         self.out = torch.zeros(9, **f64)
         self.status = torch.zeros(1, dtype=torch.int32, device=dev)
+        self.ya = torch.zeros(1, **f64)
+        self.aa = torch.zeros(1, **f64)
+        self._dot_ws = torch.empty(
+            _lib.lib().rvgp_dot_ws_bytes(self.M * self.R) // 8 + 1, **f64)
         self._cs_ws = torch.empty(
             _lib.lib().rvgp_colsumsq_ws_bytes(max(self.M, self.k), max(self.k, 1)) // 8 + 1, **f64)
         if self.form == "weight" or precision == "mixed":
@@ -80,16 +89,17 @@ class GPREngine:
             self.G = G.double().contiguous()
             self.G = 0.5 * (self.G + self.G.T)
             self.b = dense.gemm(self.Phi, self.y, transA=True)
-            self.yy = dense.gemm(self.y, self.y, transA=True).reshape(1)
+            self.yy = torch.empty(1, **f64)
+            self._dot(self.y, self.y, self.yy)
             self.A = torch.empty((self.k, self.k), **f64)
-            self.w = torch.empty((self.k, 1), **f64)
+            self.w = torch.empty((self.k, self.R), **f64)
             self.Linv = torch.empty((self.k, self.k), **f64)
             self.eye = torch.eye(self.k, **f64)
             self.ainv = torch.empty(self.k, **f64)
         if self.form == "function":
             if precision != "mixed":
                 self.K = torch.empty((self.M, self.M), **f64)
-            self.a = torch.empty((self.M, 1), **f64)
+            self.a = torch.empty((self.M, self.R), **f64)
             self.Z = torch.empty_like(self.Phi)
             self.q = torch.empty(self.k, **f64)
             self.logdet = torch.empty(1, **f64)
@@ -108,6 +118,13 @@ class GPREngine:
                   _lib.ptr(self.s2), s)
         _lib.call("rvgp_spectral_filter", self.typ, self.k, _lib.ptr(self.evals),
                   _lib.ptr(self.theta), self.n_rows, _lib.ptr(self.S), _lib.ptr(self.dS), s)
+
+    def _dot(self, x, y, out):
+        """out[0] = sum(x * y) over all entries (fixed order, FP64)."""
+        assert x.is_contiguous() and y.is_contiguous() and x.numel() == y.numel()
+        _lib.call("rvgp_dot", x.numel(), _lib.ptr(x), _lib.ptr(y), _lib.ptr(out),
+                  _lib.ptr(self._dot_ws), self._dot_ws.numel() * 8, _lib.stream())
+        return out
 
     def _colsumsq(self, Z, out):
         _lib.call("rvgp_colsumsq", int(Z.dtype == torch.float64), Z.shape[0], Z.shape[1],
@@ -134,7 +151,7 @@ class GPREngine:
             dense.gemm(self.Linv, t, self.w, transA=True)
         if self.form == "weight":
             Gw = dense.gemm(self.G, self.w)
-            _lib.call("rvgp_ws_finish", self.k, float(self.M), _lib.ptr(self.A), self.k,
+            _lib.call("rvgp_ws_finish", self.k, float(self.M), self.R, _lib.ptr(self.A), self.k,
                       _lib.ptr(self.ainv), _lib.ptr(self.b), _lib.ptr(self.w), _lib.ptr(Gw),
                       _lib.ptr(self.yy), _lib.ptr(self.S), _lib.ptr(self.dS), _lib.ptr(self.s2),
                       _lib.ptr(self.u), _lib.ptr(self.gS), _lib.ptr(self.out), s)
@@ -145,15 +162,15 @@ class GPREngine:
                 torch.div(self.y - dense.gemm(self.Phi, self.w), self.s2, out=self.a)
                 for _ in range(2):
                     self.a += self._woodbury(self.y - self._apply_Ky(self.a))
-                ya = dense.gemm(self.y, self.a, transA=True).reshape(1)
+                self._dot(self.y, self.a, self.ya)
             else:
                 self._factor()
                 # a = K_y^-1 y: L^-1 y first (ya = |L^-1 y|^2), then L^-T
                 self.a.copy_(self.y)
                 dense.trsm(self.K, self.a)
-                ya = dense.gemm(self.a, self.a, transA=True).reshape(1)
+                self._dot(self.a, self.a, self.ya)
                 dense.trsm(self.K, self.a, trans=True)
-            aa = dense.gemm(self.a, self.a, transA=True).reshape(1)
+            self._dot(self.a, self.a, self.aa)
             if self.precision == "mixed":
                 _lib.call("rvgp_lemma_terms", self.k, float(self.M), _lib.ptr(self.S),
                           _lib.ptr(self.s2), _lib.ptr(self.ainv), _lib.ptr(self.A), self.k,
@@ -162,9 +179,9 @@ class GPREngine:
                 self.Z.copy_(self.Phi)
                 dense.trsm(self.K, self.Z)
                 self._colsumsq(self.Z, self.q)
-            p = dense.gemm(self.Phi, self.a, transA=True).reshape(-1)
-            _lib.call("rvgp_fs_finish", self.k, float(self.M), _lib.ptr(self.q), _lib.ptr(p),
-                      _lib.ptr(ya), _lib.ptr(aa), _lib.ptr(self.logdet), _lib.ptr(self.S),
+            p = dense.gemm(self.Phi, self.a, transA=True)  # (k, R)
+            _lib.call("rvgp_fs_finish", self.k, float(self.M), self.R, _lib.ptr(self.q), _lib.ptr(p),
+                      _lib.ptr(self.ya), _lib.ptr(self.aa), _lib.ptr(self.logdet), _lib.ptr(self.S),
                       _lib.ptr(self.dS), _lib.ptr(self.s2), _lib.ptr(self.u), _lib.ptr(self.gS),
                       _lib.ptr(self.out), s)
         return self.out
@@ -204,14 +221,15 @@ class GPREngine:
         return float(out[0]), out[5:9].copy(), out
 
     def predict(self, Phi_star):
-        """Mean (m*,) and marginal variance (m*,) of f at rows Phi_star,
-        as GPflow's GPR.predict_f(full_cov=False)."""
+        """Mean (m*,) -- (m*, R) with R right-hand sides -- and marginal
+        variance (m*,) of f at rows Phi_star, as GPflow's
+        GPR.predict_f(full_cov=False) (the variance does not depend on y)."""
         self.evaluate()
         Ps = Phi_star.to(torch.float64).contiguous()
         m = Ps.shape[0]
         if self.form == "weight" or self.precision == "mixed":
             # mean = Phi* w, var = s2 |L_A^-1 phi*|^2 from the FP64 k x k system
-            mean = dense.gemm(Ps, self.w).reshape(-1)
+            mean = dense.gemm(Ps, self.w)
             Zt = Ps.T.contiguous()
             dense.trsm(self.A, Zt)
             v = torch.empty(m, dtype=torch.float64, device=self.dev)
@@ -222,7 +240,7 @@ class GPREngine:
             var = self.s2 * v
         else:
             p = dense.gemm(self.Phi, self.a, transA=True)
-            mean = dense.gemm(Ps, (p * self.S.reshape(-1, 1)).contiguous()).reshape(-1)
+            mean = dense.gemm(Ps, (p * self.S.reshape(-1, 1)).contiguous())
             Kms = dense.gemm(self.Phi, Ps, transB=True, kscale=self.S)  # (M, m*)
             dense.trsm(self.K, Kms)
             cs_ws = torch.empty(_lib.lib().rvgp_colsumsq_ws_bytes(max(self.M, self.k), m) // 8 + 1,
@@ -235,7 +253,7 @@ class GPREngine:
             _lib.call("rvgp_colsumsq", 1, self.k, m, _lib.ptr(Zs), m, _lib.ptr(d), _lib.ptr(cs_ws),
                       cs_ws.numel() * 8, _lib.stream())
             var = d - v
-        return mean, var
+        return (mean.reshape(-1) if self.R == 1 else mean), var
 
     # ---------------------------------------------------------------- optimisers
     def fit_lbfgs(self, maxiter=1000, disp=False):

@@ -59,6 +59,8 @@ class manifold_GPR:
         Xd = Xnew if isinstance(Xnew, torch.Tensor) else torch.as_tensor(
             np.asarray(Xnew, np.float64), device=dev)
         mean, var = self.engine.predict(Xd)
+        if self.engine.R > 1:  # (m, R): one column per right-hand side
+            return mean, var.reshape(-1, 1)
         return mean.reshape(-1, 1), var.reshape(-1, 1)
 
     def transform(self, data, test_ind):
@@ -77,6 +79,12 @@ class manifold_GPR:
         else:  # np.int64 / np.bool_ match neither branch of main.py:104-109
             raise UnboundLocalError("local variable 'test_x' referenced before assignment")
         mean, var = self.predict_f(test_x)
+        if self.engine.R > 1:
+            # a batch of R fields fitted with shared hyperparameters:
+            # mean (R, len(test_ind), D), variance (len(test_ind), D)
+            mean = mean.cpu().numpy().reshape(len(test_ind), -1, self.engine.R)
+            return np.ascontiguousarray(mean.transpose(2, 0, 1)), \
+                var.cpu().numpy().reshape(len(test_ind), -1)
         mean = mean.cpu().numpy().reshape(len(test_ind), -1)
         var = var.cpu().numpy().reshape(len(test_ind), -1)
         return mean, var
@@ -122,7 +130,13 @@ def train_gp(data, train_ind=None, n_inducing_points=None, test_size=0.2, kernel
     V = data.vectors_device()
     if V is None:
         raise TypeError("data.vectors is None: nothing to fit")
-    y = V[torch.as_tensor(nodes_tr, device=dev).long()].reshape(-1)
+    idx_tr = torch.as_tensor(nodes_tr, device=dev).long()
+    if V.dim() == 3:
+        # a batch (R, n, D) of fields observed at the same nodes: R
+        # right-hand sides of one GP with shared hyperparameters, NLML summed
+        y = V[:, idx_tr].permute(1, 2, 0).reshape(-1, V.shape[0]).contiguous()
+    else:
+        y = V[idx_tr].reshape(-1)
     noise0 = 1.0 if reference_semantics else float(noise_variance)
     lower = (kernel.nu.lower, kernel.kappa.lower, kernel.sigma_f.lower,
              GAUSSIAN_VARIANCE_LOWER_BOUND)
@@ -153,7 +167,9 @@ def train_gp(data, train_ind=None, n_inducing_points=None, test_size=0.2, kernel
 
     if len(nodes_te):
         mean, _ = model.predict_f(data.phi_rows(torch.as_tensor(nodes_te.astype(np.int32), device=dev)))
-        out_test = V[torch.as_tensor(nodes_te, device=dev).long()].reshape(-1, 1)
+        idx_te = torch.as_tensor(nodes_te, device=dev).long()
+        out_test = (V[:, idx_te].permute(1, 2, 0).reshape(-1, V.shape[0]) if V.dim() == 3
+                    else V[idx_te].reshape(-1, 1))
         # np.linalg.norm(out_test - out_pred, axis=1).mean() on (rows, 1) arrays
         err = (out_test - mean).abs().mean().item()
         model.heldout_error = err

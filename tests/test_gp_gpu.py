@@ -169,3 +169,34 @@ def test_adam_graph_replays_match_eager(form):
         _ = (torch.ones(1000, device="cuda") * step).sum().item()  # unrelated eager work
         np.testing.assert_allclose(b.u.cpu().numpy(), a.u.cpu().numpy(), rtol=1e-12, atol=1e-14)
         np.testing.assert_allclose(b.out.cpu().numpy(), a.out.cpu().numpy(), rtol=1e-11, atol=1e-12)
+
+
+@pytest.mark.parametrize("form,precision", [("function", "fp64"), ("weight", "fp64"),
+                                            ("function", "mixed"), ("weight", "mixed")])
+def test_multi_rhs_vs_oracle(form, precision):
+    """R = 9 right-hand sides sharing the hyperparameters (the multi-RHS
+    config): summed NLML, its gradient and the per-column predictive means vs
+    the oracle; the variance does not depend on Y."""
+    import torch
+    rng = np.random.default_rng(21)
+    Phi, _, lam = problem(M_nodes=64, k=30, seed=21)
+    Y = rng.normal(size=(Phi.shape[0], 9))
+    th = (1.7, 4.0, 1.2, 0.08)
+    e = engine(Phi, Y, lam, 1500.0, form, precision=precision, theta=th)
+    out = e.evaluate().cpu().numpy()
+    S, dS = cpu.spectral_filter(lam, th[0], th[1], th[2], 1500.0)
+    ref = cpu.gpr_nlml(Phi, Y, S, th[3])
+    gS, gn = cpu.gpr_grad(Phi, Y, S, th[3])
+    gref = np.array([dS[0] @ gS, dS[1] @ gS, dS[2] @ gS, gn])
+    tol = 1e-10 if precision == "fp64" else 1e-8
+    assert abs(out[0] - ref) < tol * abs(ref)
+    np.testing.assert_allclose(out[1:5], gref, rtol=1e-7 if precision == "fp64" else 1e-5,
+                               atol=1e-9 * np.abs(gref).max())
+    Ps = rng.normal(size=(25, 30))
+    m, v = e.predict(torch.as_tensor(Ps, device="cuda"))
+    assert tuple(m.shape) == (25, 9)
+    for r in (0, 4, 8):
+        mr, vr = cpu.gpr_predict(Phi, Y[:, r:r + 1], S, th[3], Ps)
+        mc = m[:, r].cpu().numpy()
+        assert np.linalg.norm(mc - mr.ravel()) / np.linalg.norm(mr) < 1e-6
+    np.testing.assert_allclose(v.cpu().numpy(), vr.ravel(), rtol=1e-6, atol=1e-10)

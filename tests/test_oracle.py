@@ -236,6 +236,30 @@ def test_gpr_nlml_vs_multivariate_normal_and_fd():
     assert abs(fd - gn) < 1e-6
 
 
+def test_gpr_multi_rhs_is_sum_of_columns():
+    """R right-hand sides with shared hyperparameters: NLML = sum of the
+    columns' -log densities (scipy), gradient = sum of the columns'
+    gradients, and central differences of the summed NLML."""
+    from scipy.stats import multivariate_normal
+    rng = np.random.default_rng(13)
+    Phi = rng.normal(size=(45, 10))
+    Y = rng.normal(size=(45, 5))
+    S = rng.uniform(0.1, 2.0, size=10)
+    cov = (Phi * S) @ Phi.T + 0.2 * np.eye(45)
+    ref = -sum(multivariate_normal(mean=np.zeros(45), cov=cov).logpdf(Y[:, r]) for r in range(5))
+    assert abs(cpu.gpr_nlml(Phi, Y, S, 0.2) - ref) < 1e-9 * abs(ref)
+    gS, gn = cpu.gpr_grad(Phi, Y, S, 0.2)
+    cols = [cpu.gpr_grad(Phi, Y[:, r:r + 1], S, 0.2) for r in range(5)]
+    np.testing.assert_allclose(gS, sum(c[0] for c in cols), rtol=1e-10)
+    assert abs(gn - sum(c[1] for c in cols)) < 1e-10 * abs(gn)
+    h = 1e-6
+    Sp, Sm = S.copy(), S.copy()
+    Sp[3] += h
+    Sm[3] -= h
+    fd = (cpu.gpr_nlml(Phi, Y, Sp, 0.2) - cpu.gpr_nlml(Phi, Y, Sm, 0.2)) / (2 * h)
+    assert abs(fd - gS[3]) < 1e-5
+
+
 def test_gpr_predict_conditioning_identity():
     """Predicting at the training inputs with noise -> 0 interpolates y."""
     rng = np.random.default_rng(4)

@@ -211,6 +211,172 @@ def roofline_only(args):
                       "GBps": nbytes / (us * 1e3), "copy_bytes_each_way": 1 << 30}))
 
 
+# ---------------------------------------------------------------- other configs
+def _timed(fn):
+    import torch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r = fn()
+    torch.cuda.synchronize()
+    return r, time.perf_counter() - t0
+
+
+def bench_batched(args):
+    """Batched fits on one shared 3-torus manifold: n = 50,000 (the 3-torus
+    in R^32 generator), dim_man = 3, 300 eigenpairs; 256 Gaussian tangent
+    fields smoothed with diffusion times log-uniform in [10, 500]; each fit
+    has its own training subset of uniform size in [100, 5000] nodes and its
+    own hyperparameters; 200 Adam steps, every step of all 256 fits one
+    graph-replayed launch.  Timed: the batched fit (statistics + 200 steps)
+    on the resident data object and fields; beside it, the same fits run one
+    after another through the single-fit engine (a sample of 8 fits, scaled
+    to 256)."""
+    import numpy as np
+    import torch
+    import rvgp_amd as RVGP
+    from rvgp_amd import _lib
+    from rvgp_amd.batch import BatchedGPR
+    from rvgp_amd.gp import GPREngine
+    _lib.lib()
+    dev = _lib.device()
+    n, F, k_eig, steps = args.batch_n, args.batch_fields, 300, args.adam_steps
+    X, _, _ = flat_torus3(n, seed=0)
+    d = RVGP.create_data_object(torch.as_tensor(X, device=dev), n_neighbors=args.k_nn,
+                                n_eigenpairs=k_eig, precision="mixed", reference_semantics=False,
+                                verbose=False)
+    rng = np.random.default_rng(7)
+    ts = np.exp(rng.uniform(np.log(10.0), np.log(500.0), F))
+    V = torch.as_tensor(rng.normal(size=(F, n, X.shape[1])), device=dev)
+    d.vectors = d._frame(2, V, X.shape[1])
+    del V
+    (_, t_smooth) = _timed(lambda: d.smooth_vector_field(ts))
+    log(f"[bench batched] n={n} dim_man={d.dim_man} fields={F}: smoothing {t_smooth:.2f}s")
+    sizes = rng.integers(100, 5001, F)
+    sets = [rng.choice(n, size=int(m), replace=False) for m in sizes]
+    th0 = np.stack([rng.uniform(1.0, 2.5, F), rng.uniform(3.0, 8.0, F), np.ones(F),
+                    np.full(F, 0.1)], 1)
+    Vd = d.vectors_device()
+    RVGP.fit_batched(d, sets, vectors=Vd, epochs=2, theta0=th0)  # warm-up (and autotune)
+    gp, t_fit = _timed(lambda: RVGP.fit_batched(d, sets, vectors=Vd, epochs=steps, theta0=th0))
+    gp.check_status()
+    # per-step launch time over graph replays (HIP events on the launch stream)
+    g = gp.capture_adam_step(0.05)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(50):
+        g.replay()
+    e1.record()
+    e1.synchronize()
+    step_ms = e0.elapsed_time(e1) / 50
+    e0.record()
+    gp2 = BatchedGPR(d, Vd, sets, theta0=th0, gram_tile=gp.gram_tile, panel=gp.panel)
+    e1.record()
+    e1.synchronize()
+    setup_ms = e0.elapsed_time(e1)
+    del gp2
+    k = gp.k
+    rows = int(sizes.sum()) * d.dim_man
+    step_flops = F * 2.0 * k ** 3 / 3.0  # Cholesky + triangular inverse per fit
+    gram_flops = rows * float(k) * k     # symmetric Gram over the gathered intrinsic rows
+    # sequential single fits through the single-fit weight-space engine
+    samp = list(range(0, F, F // 8))[:8]
+    lower = gp.lower.cpu().numpy()
+
+    def seq():
+        for f in samp:
+            nodes = torch.as_tensor(sets[f].astype(np.int32), device=dev)
+            Phi = d.phi_rows(nodes)
+            y = Vd[f, nodes.long()].reshape(-1)
+            e = GPREngine(Phi, y, d.spectrum_device(), d.n * X.shape[1], form="weight",
+                          precision="fp64", lower=lower)
+            e.set_theta(th0[f])
+            e.adam_setup()
+            gg = e.capture_adam_step(lr=0.05)
+            for _ in range(steps):
+                gg.replay()
+    _, t_seq = _timed(seq)
+    t_seq_all = t_seq * F / len(samp)
+    out = {
+        "metric": "batched fit wall-time: 256 fields x 200 adjoint Adam steps on one manifold",
+        "value": round(t_fit, 4), "unit": "s", "n_gpus": 1, "higher_is_better": False,
+        "dtype": "f64", "data": "synthetic (seeded flat 3-torus in R^32, smoothed Gaussian fields)",
+        "config": {"workload": (f"3-torus in R^32, n={n}, k={args.k_nn}, dim_man={d.dim_man}, "
+                                f"{k_eig} eigenpairs, {F} fields, subsets 100-5000 nodes, "
+                                f"{steps} Adam steps"), "fields": F, "n": n},
+        "fits_steps_per_s": round(F * steps / t_fit, 1),
+        "step_ms": round(step_ms, 4), "setup_ms": round(setup_ms, 3),
+        "step_fp64_tflops": round(step_flops / (step_ms * 1e9), 3),
+        "gram_rows": rows, "gram_fp64_tflops_incl_rhs": round(gram_flops / (setup_ms * 1e9), 3),
+        "autotune": {"gram_tile": gp.gram_tile, "panel": gp.panel},
+        "sequential_single_fit_s": round(t_seq_all, 3),
+        "speedup_vs_sequential": round(t_seq_all / t_fit, 2),
+        "smoothing_s": round(t_smooth, 3),
+    }
+    print(json.dumps(out))
+
+
+def bench_multirhs(args):
+    """Scalp-like multi-RHS: n = 5,000 points by FPS from 50,000 on the upper
+    half of the ellipsoid (1, 0.9, 0.8), k = 12, dim_man = 2, 150
+    eigenpairs; 2,000 fields = 3 smoothed Gaussian tangent modes (t = 20)
+    with AR(1) time courses (rho = 0.95, unit stationary variance), observed
+    at a fixed set of 64 nodes: one GP with 2,000 right-hand sides and shared
+    hyperparameters (summed NLML), 200 Adam steps, then prediction at every
+    node.  Timed: fit + prediction on the resident data object."""
+    import numpy as np
+    import torch
+    import rvgp_amd as RVGP
+    from rvgp_amd import _lib, geometry
+    _lib.lib()
+    dev = _lib.device()
+    rng = np.random.default_rng(0)
+    P = rng.normal(size=(50000, 3))
+    P /= np.linalg.norm(P, axis=1, keepdims=True)
+    P[:, 2] = np.abs(P[:, 2])
+    P *= np.array([1.0, 0.9, 0.8])
+    perm, _ = geometry.furthest_point_sampling(P, N=5000)
+    X = P[perm]
+    d = RVGP.create_data_object(X, n_neighbors=12, dim_man=2, n_eigenpairs=150,
+                                precision=args.precision_multirhs, verbose=False)
+    n, D = X.shape
+    modes = d._frame(2, torch.as_tensor(rng.normal(size=(3, n, D)), device=dev), D)
+    d.vectors = modes
+    d.smooth_vector_field(20.0)
+    modes = d.vectors_device()
+    T, rho = args.multirhs_T, 0.95
+    a = np.zeros((T, 3))
+    a[0] = rng.normal(size=3)
+    for t in range(1, T):
+        a[t] = rho * a[t - 1] + np.sqrt(1 - rho ** 2) * rng.normal(size=3)
+    fields = torch.einsum("tm,mnd->tnd", torch.as_tensor(a, device=dev), modes).contiguous()
+    d.vectors = fields
+    obs = rng.choice(n, size=64, replace=False)
+    test = [int(i) for i in range(n)]
+
+    def run():
+        gp = RVGP.fit(d, train_ind=obs, epochs=args.adam_steps, optimizer="adam", lr=0.05,
+                      reference_semantics=False, precision=args.precision_multirhs, verbose=False)
+        mean, var = gp.transform(d, test)
+        return gp, mean, var
+    run()  # warm-up
+    (gp, mean, var), t = _timed(run)
+    err = float(np.linalg.norm(mean[:, obs] - fields[:, obs].cpu().numpy())
+                / np.linalg.norm(fields[:, obs].cpu().numpy()))
+    out = {
+        "metric": "multi-RHS fit + predict wall-time: 2000 fields at 64 observed nodes",
+        "value": round(t, 4), "unit": "s", "n_gpus": 1, "higher_is_better": False,
+        "dtype": "f64" if args.precision_multirhs == "fp64" else "f32+f64",
+        "data": "synthetic (FPS half-ellipsoid, 3 smoothed modes x AR(1) time courses)",
+        "config": {"workload": (f"half-ellipsoid n={n}, k=12, dim_man={d.dim_man}, 150 eigenpairs, "
+                                f"{T} right-hand sides at 64 nodes, {args.adam_steps} Adam steps, "
+                                f"prediction at all {n} nodes")},
+        "fit_form": gp.engine.form, "rhs": gp.engine.R, "observations": gp.engine.M,
+        "hyperparameters": {k: round(float(v), 5) for k, v in gp.hyperparameters.items()},
+        "rel_err_at_observed_nodes": err,
+    }
+    print(json.dumps(out))
+
+
 # ---------------------------------------------------------------- main
 def main():
     p = argparse.ArgumentParser()
@@ -230,6 +396,14 @@ def main():
     p.add_argument("--cpu-k-eig", dest="cpu_k_eig", type=int, default=50)
     p.add_argument("--verbose", action="store_true")
     p.add_argument("--roofline-only", dest="roofline_only", action="store_true")
+    p.add_argument("--config", default="fit", choices=["fit", "batched", "multirhs"],
+                   help="fit: the headline full fit (default); batched: 256 fits on one "
+                        "manifold; multirhs: 2000 right-hand sides at 64 nodes")
+    p.add_argument("--batch-n", dest="batch_n", type=int, default=50000)
+    p.add_argument("--batch-fields", dest="batch_fields", type=int, default=256)
+    p.add_argument("--multirhs-T", dest="multirhs_T", type=int, default=2000)
+    p.add_argument("--precision-multirhs", dest="precision_multirhs", default="fp64",
+                   choices=["fp64", "mixed"])
     args = p.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
@@ -237,6 +411,12 @@ def main():
         sys.exit(2)
     if args.roofline_only:
         roofline_only(args)
+        return
+    if args.config == "batched":
+        bench_batched(args)
+        return
+    if args.config == "multirhs":
+        bench_multirhs(args)
         return
 
     import numpy as np

@@ -37,11 +37,32 @@ def cheb_coeffs(t, ub, rtol=1e-17, max_terms=200000):
 
 
 def expm_multiply(A: BSR, X, t, ub=None):
-    """exp(-t A) X for a PSD BSR operator A (X: (rows,) or (rows, r))."""
+    """exp(-t A) X for a PSD BSR operator A (X: (rows,) or (rows, r)).  t is
+    a scalar, or one diffusion time per column of X: the recurrence T_j X
+    then runs once, to the longest series, and each column accumulates its
+    own coefficients."""
     if ub is None:
         ub = A.gershgorin_bound() * 1.0001 + 1e-12
-    c = cheb_coeffs(t, ub)
     X2 = X.reshape(-1, 1) if X.dim() == 1 else X
+    ts = np.atleast_1d(np.asarray(t, dtype=np.float64))
+    if ts.size > 1:
+        if ts.size != X2.shape[1]:
+            raise ValueError(f"{ts.size} diffusion times for {X2.shape[1]} columns")
+        cs = [cheb_coeffs(tc, ub) for tc in ts]
+        C = np.zeros((max(len(c) for c in cs), ts.size))
+        for col, c in enumerate(cs):
+            C[:len(c), col] = c
+        Cd = torch.as_tensor(C, dtype=X2.dtype, device=X2.device)
+        out = X2 * Cd[0]
+        T1 = spmm(A, X2, alpha=2.0 / ub, beta=-1.0)
+        out.addcmul_(T1, Cd[1])
+        prev, cur = X2.clone(), T1
+        for j in range(2, C.shape[0]):
+            spmm(A, cur, Y=prev, alpha=4.0 / ub, beta=-2.0, Z=prev, gamma=-1.0)
+            out.addcmul_(prev, Cd[j])
+            prev, cur = cur, prev
+        return out.reshape(X.shape)
+    c = cheb_coeffs(float(ts[0]), ub)
     out = X2 * float(c[0])
     if len(c) > 1:
         # T_1 = (2/ub) A X - X
@@ -58,21 +79,29 @@ def expm_multiply(A: BSR, X, t, ub=None):
 def vector_diffusion_device(c, t, Lc: BSR, L: BSR):
     """smoothing.py:37-64 (method="matrix_exp", normalise=True) on device.
 
-    c: (n, d) local-frame coefficients, or (F, n, d) for F fields at once."""
+    c: (n, d) local-frame coefficients, or (F, n, d) for F fields at once;
+    t: a diffusion time, or (F,) times, one per field."""
     batched = c.dim() == 3
     C = c if batched else c.unsqueeze(0)
     F, n, d = C.shape
+    ts = np.atleast_1d(np.asarray(t, dtype=np.float64))
+    per_field = ts.size > 1
+    if per_field and ts.size != F:
+        raise ValueError(f"{ts.size} diffusion times for {F} fields")
     # connection diffusion of every field: columns = fields
     Xc = C.reshape(F, n * d).T.contiguous()
-    out = expm_multiply(Lc, Xc, t).T.reshape(F, n, d).contiguous()
+    out = expm_multiply(Lc, Xc, ts if per_field else float(ts[0])).T.reshape(F, n, d).contiguous()
     # scalar diffusions exp(-tL)|x| and exp(-tL)1 share one multi-column pass
-    xs = torch.empty((n, F + 1), dtype=torch.float64, device=c.device)
+    # (one column of ones per distinct time)
+    n1 = F if per_field else 1
+    xs = torch.empty((n, F + n1), dtype=torch.float64, device=c.device)
     xs[:, :F] = torch.linalg.vector_norm(C, dim=2).T
-    xs[:, F] = 1.0
-    sd = expm_multiply(L, xs, t)
+    xs[:, F:] = 1.0
+    sd = expm_multiply(L, xs, np.concatenate([ts, ts]) if per_field else float(ts[0]))
     for f in range(F):
-        _lib.call("rvgp_diffusion_normalise", n, d, _lib.ptr(out[f]), d, _lib.ptr(sd[:, f:]), F + 1,
-                  _lib.ptr(sd[:, F:]), F + 1, _lib.stream())
+        one = F + (f if per_field else 0)
+        _lib.call("rvgp_diffusion_normalise", n, d, _lib.ptr(out[f]), d, _lib.ptr(sd[:, f:]),
+                  F + n1, _lib.ptr(sd[:, one:]), F + n1, _lib.stream())
     return out if batched else out[0]
 
 

@@ -183,3 +183,30 @@ def test_vector_diffusion_vs_oracle():
     for f in range(2):
         ref = cpu.vector_diffusion(cl2[f], 3.0, Lc, L, "expm")
         assert np.abs(outb[f] - ref).max() < 1e-10 * np.abs(ref).max()
+
+
+def test_vector_diffusion_per_field_times_vs_oracle():
+    """One diffusion time per field (the batched-fits config draws them
+    log-uniformly): one shared Chebyshev recurrence, per-column coefficients,
+    each field equal to the dense-expm oracle at its own time."""
+    import torch
+    from rvgp_amd.smoothing import vector_diffusion_device
+    X = sphere(250, 6)
+    rp, c = cpu.sym_csr(cpu.knn(X, 10))
+    T, S = cpu.tangent_frames(X, rp, c, 3, 15)
+    T2 = np.ascontiguousarray(T[:, :, :2])
+    R = cpu.connections(T2, rp, c)
+    Lc = cpu.connection_laplacian(rp, c, R)
+    L = cpu.laplacian(rp, c)
+    A = _bsr_from_blocks(rp, c, cpu.lc_blocks(rp, c, R), 2)
+    rows = np.repeat(np.arange(250), np.diff(rp))
+    A1 = _bsr_from_blocks(rp, c, np.asarray(L[rows, c]).ravel(), 1)
+    rng = np.random.default_rng(8)
+    ts = np.array([0.5, 3.0, 12.0, 40.0])
+    cl = np.stack([cpu.express_in_local_frame(rng.normal(size=(250, 3)), T2) for _ in ts])
+    out = vector_diffusion_device(torch.as_tensor(cl, device="cuda"), ts, A, A1).cpu().numpy()
+    for f, t in enumerate(ts):
+        ref = cpu.vector_diffusion(cl[f], t, Lc, L, "expm")
+        raw = cpu.heat(Lc, t, cl[f].reshape(-1, 1), "eigh")
+        amp = np.linalg.norm(cl[f]) / np.linalg.norm(raw)
+        assert np.abs(out[f] - ref).max() < 1e-12 * amp * np.abs(ref).max(), (f, t)

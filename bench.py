@@ -377,6 +377,133 @@ def bench_multirhs(args):
     print(json.dumps(out))
 
 
+def bench_sphere(args):
+    """Plain method on S^2, FP64: n = 2,000 points uniform on the unit sphere
+    (seed 0), k = 10, dim_man = 2, 50 eigenpairs; standard-normal ambient
+    vectors projected onto the tangent planes and smoothed with t = 100;
+    train on a random 50 % (the reference's 80/20 re-split applies), noise
+    variance 1e-3, Matern defaults, L-BFGS-B; predict the other 50 %.  Timed:
+    the whole pipeline on the GPU (data object, field, smoothing, fit,
+    transform), warm.  Beside it the oracle: data object and dense-expm
+    smoothing timed in full, NLML + gradient timed over 3 evaluations and
+    scaled to the GPU fit's L-BFGS-B evaluation count."""
+    import numpy as np
+    import torch
+    import rvgp_amd as RVGP
+    from rvgp_amd import _lib
+    from oracle import cpu
+    _lib.lib()
+    dev = _lib.device()
+    rng = np.random.default_rng(0)
+    X = rng.normal(size=(2000, 3))
+    X /= np.linalg.norm(X, axis=1, keepdims=True)
+    v0 = rng.normal(size=(2000, 3))
+    train = rng.permutation(2000)[:1000]
+    test = [int(i) for i in np.setdiff1d(np.arange(2000), train)]
+
+    def run():
+        d = RVGP.create_data_object(X, n_neighbors=10, dim_man=2, n_eigenpairs=50, verbose=False)
+        d.vectors = d._frame(2, torch.as_tensor(v0, device=dev), 3)
+        d.smooth_vector_field(t=100)
+        gp = RVGP.fit(d, train_ind=train, noise_variance=1e-3, verbose=False)
+        mean, var = gp.transform(d, test)
+        return d, gp, mean, var
+    run()
+    (d, gp, mean, var), t_gpu = _timed(run)
+    nfev = int(getattr(gp.engine, "last_nfev", 0)) or None
+    # oracle on the same input; prediction parity at the GPU's hyperparameters
+    t0 = time.perf_counter()
+    o = cpu.Data(X, n_neighbors=10, n_eigenpairs=50)
+    o.vectors = cpu.project_to_manifold(v0, o.gauges)
+    o.smooth_vector_field(t=100)
+    t_data = time.perf_counter() - t0
+    tr, _ = cpu.train_test_split_indices(len(train), 0.2, 0)
+    nodes = train[tr]
+    k = o.evecs_Lc.shape[1]
+    Phi = o.evecs_Lc.reshape(2000, -1)[nodes].reshape(-1, k)
+    y = o.vectors[nodes].reshape(-1, 1)
+    th = gp.engine.theta_host()
+    S, dS = cpu.spectral_filter(o.evals_Lc, th[0], th[1], th[2], 2000 * 3)
+    t0 = time.perf_counter()
+    for _ in range(3):
+        cpu.gpr_nlml(Phi, y, S, th[3])
+        cpu.gpr_grad(Phi, y, S, th[3])
+    t_eval = (time.perf_counter() - t0) / 3
+    Ps = o.evecs_Lc.reshape(2000, -1)[np.asarray(test)].reshape(-1, k)
+    mref, _ = cpu.gpr_predict(Phi, y, S, th[3], Ps)
+    rel = float(np.linalg.norm(mean.ravel() - mref.ravel()) / np.linalg.norm(mref))
+    evals_rel = float(np.max(np.abs(d.evals_Lc - o.evals_Lc) / np.maximum(np.abs(o.evals_Lc), 1e-9)))
+    out = {
+        "metric": "full pipeline wall-time, S^2 n=2000, 50 eigenpairs, FP64 (drop-in defaults)",
+        "value": round(t_gpu, 4), "unit": "s", "n_gpus": 1, "higher_is_better": False,
+        "dtype": "f64", "data": "synthetic (seeded uniform S^2, projected normal field)",
+        "config": {"workload": "S^2 n=2000, k=10, dim_man=2, 50 eigenpairs, t=100, 50% train, "
+                               "L-BFGS-B (maxiter 1000), predict 50%"},
+        "lbfgs_evaluations": nfev,
+        "pred_rel_err_vs_oracle": rel, "evals_Lc_rel_err_vs_oracle": evals_rel,
+        "cpu_baseline": {"value": round(t_data + (nfev or 1) * t_eval, 3), "unit": "s",
+                         "kind": "port", "cores": None,
+                         "sample": (f"oracle data object + dense-expm smoothing {t_data:.2f}s in full; "
+                                    f"NLML+gradient {t_eval:.3f}s per evaluation x "
+                                    f"{nfev} L-BFGS-B evaluations")},
+    }
+    print(json.dumps(out))
+
+
+def bench_torus(args):
+    """Torus of radii R = 2, r = 0.7: 20,000 points from uniform (theta, phi),
+    FPS with stop_crit 0.015 (~3,956 points), k = 15, dim_man = 2, 200
+    eigenpairs; a Gaussian tangent field smoothed with t = 50 plus
+    N(0, 0.05^2) ambient noise; 50 % of the nodes train; 200 Adam steps on
+    the NLML through the adjoint.  Timed (warm): FPS, data object, field
+    smoothing and fit, in the mixed form and in the FP64 form."""
+    import numpy as np
+    import torch
+    import rvgp_amd as RVGP
+    from rvgp_amd import _lib, geometry
+    _lib.lib()
+    dev = _lib.device()
+    rng = np.random.default_rng(0)
+    th, ph = rng.uniform(0, 2 * np.pi, (2, 20000))
+    P = np.stack([(2 + 0.7 * np.cos(ph)) * np.cos(th), (2 + 0.7 * np.cos(ph)) * np.sin(th),
+                  0.7 * np.sin(ph)], 1)
+    g0 = rng.normal(size=(20000, 3))
+    noise = 0.05 * rng.normal(size=(20000, 3))
+    res = {}
+
+    def run(prec):
+        perm, _ = geometry.furthest_point_sampling(P, spacing=0.015)
+        X = P[perm]
+        d = RVGP.create_data_object(X, n_neighbors=15, dim_man=2, n_eigenpairs=200,
+                                    precision=prec, verbose=False)
+        n = len(perm)
+        d.vectors = d._frame(2, torch.as_tensor(g0[:n], device=dev), 3)
+        d.smooth_vector_field(t=50)
+        d.vectors = d.vectors_device() + torch.as_tensor(noise[:n], device=dev)
+        train = np.random.default_rng(3).permutation(n)[: n // 2]
+        gp = RVGP.fit(d, train_ind=train, epochs=200, optimizer="adam", lr=0.05, precision=prec,
+                      reference_semantics=False, verbose=False)
+        test = [int(i) for i in np.setdiff1d(np.arange(n), train)]
+        mean, _ = gp.transform(d, test)
+        return n, mean
+    for prec in ("mixed", "fp64"):
+        run(prec)
+        (n, mean), t = _timed(lambda: run(prec))
+        res[prec] = (t, mean)
+    rel = float(np.linalg.norm(res["mixed"][1] - res["fp64"][1]) / np.linalg.norm(res["fp64"][1]))
+    out = {
+        "metric": "FPS + full fit wall-time, torus n=20k -> FPS(0.015), 200 eigenpairs, 200 Adam steps",
+        "value": round(res["mixed"][0], 4), "unit": "s", "n_gpus": 1, "higher_is_better": False,
+        "dtype": "f32+f64", "data": "synthetic (seeded torus R=2 r=0.7, smoothed Gaussian field + noise)",
+        "config": {"workload": (f"torus R=2 r=0.7, 20000 points -> FPS(0.015) = {n} points, k=15, "
+                                f"dim_man=2, 200 eigenpairs, t=50, 50% train, 200 Adam steps")},
+        "fp64_s": round(res["fp64"][0], 4),
+        "speedup_vs_fp64": round(res["fp64"][0] / res["mixed"][0], 3),
+        "rel_err_pred_mixed_vs_fp64": rel,
+    }
+    print(json.dumps(out))
+
+
 # ---------------------------------------------------------------- main
 def main():
     p = argparse.ArgumentParser()
@@ -396,9 +523,11 @@ def main():
     p.add_argument("--cpu-k-eig", dest="cpu_k_eig", type=int, default=50)
     p.add_argument("--verbose", action="store_true")
     p.add_argument("--roofline-only", dest="roofline_only", action="store_true")
-    p.add_argument("--config", default="fit", choices=["fit", "batched", "multirhs"],
+    p.add_argument("--config", default="fit",
+                   choices=["fit", "batched", "multirhs", "sphere", "torus"],
                    help="fit: the headline full fit (default); batched: 256 fits on one "
-                        "manifold; multirhs: 2000 right-hand sides at 64 nodes")
+                        "manifold; multirhs: 2000 right-hand sides at 64 nodes; sphere: the "
+                        "plain FP64 method on S^2 n=2000; torus: FPS torus, 200 eigenpairs")
     p.add_argument("--batch-n", dest="batch_n", type=int, default=50000)
     p.add_argument("--batch-fields", dest="batch_fields", type=int, default=256)
     p.add_argument("--multirhs-T", dest="multirhs_T", type=int, default=2000)
@@ -417,6 +546,12 @@ def main():
         return
     if args.config == "multirhs":
         bench_multirhs(args)
+        return
+    if args.config == "sphere":
+        bench_sphere(args)
+        return
+    if args.config == "torus":
+        bench_torus(args)
         return
 
     import numpy as np
@@ -464,8 +599,24 @@ def main():
         w64 = d64._evals_Lc.cpu().numpy()
         wmx = d_mx._evals_Lc.cpu().numpy()
         ub = d64._Lcb.gershgorin_bound()
+        # eigenspace principal angles between the mixed and FP64 runs over
+        # clusters of the FP64 eigenvalues delimited by relative gaps > 1e-5
+        # (the basis inside a degenerate cluster is free); the last cluster
+        # is skipped when it may continue past the k-th eigenvalue
+        import torch
+        from rvgp_amd import dense
+        rows = d64._U.shape[0]
+        Mx = dense.gemm(d64._U, d_mx._U, transA=True).cpu().numpy() / rows
+        gaps = np.diff(w64) > 1e-5 * np.maximum(np.abs(w64[1:]), 1e-9 * ub)
+        cuts = [0] + list(np.nonzero(gaps)[0] + 1) + [len(w64)]
+        ang = 0.0
+        for a0, a1 in zip(cuts[:-2], cuts[1:-1]):
+            sv = np.linalg.svd(Mx[a0:a1, a0:a1], compute_uv=False)
+            ang = max(ang, float(np.arcsin(np.sqrt(max(0.0, 1.0 - float(sv.min()) ** 2)))))
         result_extra = {
             "fp64_s": t_fp64,
+            "eig_angle_max_rad": ang,
+            "eig_clusters": len(cuts) - 2,
             "speedup_vs_fp64": t_fp64 / t_mixed,
             "rel_err_pred": float(np.linalg.norm(f_mx - f64) / np.linalg.norm(f64)),
             "eig_rel_err_max": float(np.max(np.abs(wmx - w64) / np.maximum(np.abs(w64), 1e-9 * ub))),
@@ -506,6 +657,8 @@ def main():
                    "adam_steps": args.adam_steps, "parallelism": "single GPU"},
         **result_extra,
         "stages_mixed_s": {k: round(v, 4) for k, v in stages.items()},
+        "gauges_connections_points_per_s": round(
+            args.n / max(1e-9, stages.get("gauges", 0.0) + stages.get("connections", 0.0)), 1),
         "eig_stats_mixed": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv)
                                 for kk, vv in v.items()} for k, v in d_mx.eig_stats.items()},
         "roofline": roof,

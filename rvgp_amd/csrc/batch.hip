@@ -364,48 +364,79 @@ __global__ __launch_bounds__(ST) void batch_step_kernel(
             // (3) trailing update on the matrix cores, 16 x 16 tiles:
             //     A[i, j] -= P_i . P_j   (t0 <= j <= i: lower tiles)
             //     X[i, c] -= P_i . X[J, c]  (c < t0)
+            // A wave takes TG tiles at a time and issues all their loads of
+            // the old values before the products, so TG round trips to the
+            // cache overlap instead of following one another.
             const int TR = (nrow + 15) / 16, TC = (ncol + 15) / 16;
             const int nA = TR * (TR + 1) / 2, ntile = nA + TR * TC;
-            for (int tI = wv; tI < ntile; tI += ST / 64) {
-                int ti, tj;
-                bool isA = tI < nA;
-                if (isA) {
-                    // lower-triangle tile index -> (ti, tj), tj <= ti
-                    ti = (int)((sqrt(8.0 * tI + 1.0) - 1.0) * 0.5);
-                    while ((ti + 1) * (ti + 2) / 2 <= tI) ++ti;
-                    while (ti * (ti + 1) / 2 > tI) --ti;
-                    tj = tI - ti * (ti + 1) / 2;
-                } else {
-                    const int q = tI - nA;
-                    ti = q / TC;
-                    tj = q - ti * TC;
-                }
-                dbl4 acc = dbl4{0, 0, 0, 0};
-                const int ar = ti * 16 + (lane & 15);  // panel row of the A fragment
+            constexpr int TG = 4;
+            for (int base = wv * TG; base < ntile; base += (ST / 64) * TG) {
+                int tis[TG], tjs[TG];
+                dbl4 cv[TG];
 #pragma unroll
-                for (int ks = 0; ks < NB; ks += 4) {
-                    const int kc = ks + (lane >> 4);
-                    const double a = (ar < nrow && kc < jb) ? P[ar * (NB + 1) + kc] : 0.0;
-                    double b;
-                    if (isA) {
-                        const int br = tj * 16 + (lane & 15);
-                        b = (br < nrow && kc < jb) ? P[br * (NB + 1) + kc] : 0.0;
-                    } else {
-                        const int bc = tj * 16 + (lane & 15);
-                        b = (bc < ncol && kc < jb) ? W[(size_t)(j0 + kc) * ldw + k + bc] : 0.0;
+                for (int g = 0; g < TG; ++g) {
+                    const int tI = base + g;
+                    int ti = -1, tj = 0;
+                    if (tI < nA) {
+                        // lower-triangle tile index -> (ti, tj), tj <= ti
+                        ti = (int)((sqrt(8.0 * tI + 1.0) - 1.0) * 0.5);
+                        while ((ti + 1) * (ti + 2) / 2 <= tI) ++ti;
+                        while (ti * (ti + 1) / 2 > tI) --ti;
+                        tj = tI - ti * (ti + 1) / 2;
+                    } else if (tI < ntile) {
+                        const int q = tI - nA;
+                        ti = q / TC;
+                        tj = TR + q - (q / TC) * TC;  // X tiles: tj >= TR marks them
                     }
-                    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+                    tis[g] = ti;
+                    tjs[g] = tj;
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int row = ti * 16 + (lane >> 4) + 4 * r;
+                        double v = 0.0;
+                        if (ti >= 0 && row < nrow) {
+                            if (tj < TR) {
+                                const int col = tj * 16 + (lane & 15);
+                                if (col <= row) v = W[(size_t)(t0 + row) * ldw + t0 + col];
+                            } else {
+                                const int col = (tj - TR) * 16 + (lane & 15);
+                                if (col < ncol) v = W[(size_t)(t0 + row) * ldw + k + col];
+                            }
+                        }
+                        cv[g][r] = v;
+                    }
                 }
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int row = ti * 16 + (lane >> 4) + 4 * r;
-                    const int col = tj * 16 + (lane & 15);
-                    if (row < nrow) {
-                        if (isA) {
-                            if (col <= row)
-                                W[(size_t)(t0 + row) * ldw + t0 + col] -= acc[r];
-                        } else if (col < ncol) {
-                            W[(size_t)(t0 + row) * ldw + k + col] -= acc[r];
+                for (int g = 0; g < TG; ++g) {
+                    const int ti = tis[g], tj = tjs[g];
+                    if (ti < 0) continue;
+                    dbl4 acc = dbl4{0, 0, 0, 0};
+                    const int ar = ti * 16 + (lane & 15);  // panel row of the A fragment
+#pragma unroll
+                    for (int ks = 0; ks < NB; ks += 4) {
+                        const int kc = ks + (lane >> 4);
+                        const double a = (ar < nrow && kc < jb) ? P[ar * (NB + 1) + kc] : 0.0;
+                        double b;
+                        if (tj < TR) {
+                            const int br = tj * 16 + (lane & 15);
+                            b = (br < nrow && kc < jb) ? P[br * (NB + 1) + kc] : 0.0;
+                        } else {
+                            const int bc = (tj - TR) * 16 + (lane & 15);
+                            b = (bc < ncol && kc < jb) ? W[(size_t)(j0 + kc) * ldw + k + bc] : 0.0;
+                        }
+                        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+                    }
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int row = ti * 16 + (lane >> 4) + 4 * r;
+                        if (row >= nrow) continue;
+                        const double v = cv[g][r] - acc[r];
+                        if (tj < TR) {
+                            const int col = tj * 16 + (lane & 15);
+                            if (col <= row) W[(size_t)(t0 + row) * ldw + t0 + col] = v;
+                        } else {
+                            const int col = (tj - TR) * 16 + (lane & 15);
+                            if (col < ncol) W[(size_t)(t0 + row) * ldw + k + col] = v;
                         }
                     }
                 }
@@ -415,26 +446,46 @@ __global__ __launch_bounds__(ST) void batch_step_kernel(
         bad = (int)wave_max((double)bad);
         if (wv == 0 && lane == 0 && status) status[f] = bad ? RVGP_ST_NOT_SPD : RVGP_ST_OK;
 
-        // ---- diag(A^-1) = column norms of X; t = X b
-        for (int j = tid; j < k; j += ST) {
-            double s = 0.0;
-            for (int i = j; i < k; ++i) {
-                const double x = W[(size_t)i * ldw + k + j];
-                s += x * x;
-            }
-            ainv[j] = s;
-        }
+        // ---- one coalesced pass over the rows of X = L^-1 (a wave per row):
+        //      t = X b, and the column norms diag(A^-1)_c = sum_i X[i][c]^2
+        //      accumulated per wave in LDS (the panel's space), then summed
+        //      in wave order
+        double* part = P;  // (ST / 64) x k partial sums
+        for (int e = tid; e < (ST / 64) * k; e += ST) part[e] = 0.0;
+        __syncthreads();
         for (int i = wv; i < k; i += ST / 64) {
             double s = 0.0;
-            for (int c = lane; c <= i; c += 64) s += W[(size_t)i * ldw + k + c] * bf[c];
+            const double* xr = W + (size_t)i * ldw + k;
+            for (int c = lane; c <= i; c += 64) {
+                const double x = xr[c];
+                s += x * bf[c];
+                part[wv * k + c] += x * x;
+            }
             s = wave_sum(s);
             if (lane == 0) tv[i] = s;
         }
         __syncthreads();
-        // ---- w = X^T t
         for (int c = tid; c < k; c += ST) {
             double s = 0.0;
-            for (int i = c; i < k; ++i) s += W[(size_t)i * ldw + k + c] * tv[i];
+#pragma unroll
+            for (int q = 0; q < ST / 64; ++q) {
+                s += part[q * k + c];
+                part[q * k + c] = 0.0;
+            }
+            ainv[c] = s;
+        }
+        __syncthreads();
+        // ---- w = X^T t, the same row pass
+        for (int i = wv; i < k; i += ST / 64) {
+            const double ti = tv[i];
+            const double* xr = W + (size_t)i * ldw + k;
+            for (int c = lane; c <= i; c += 64) part[wv * k + c] += xr[c] * ti;
+        }
+        __syncthreads();
+        for (int c = tid; c < k; c += ST) {
+            double s = 0.0;
+#pragma unroll
+            for (int q = 0; q < ST / 64; ++q) s += part[q * k + c];
             wv_[c] = s;
         }
         __syncthreads();
@@ -510,7 +561,11 @@ __global__ __launch_bounds__(ST) void batch_step_kernel(
 
 template <int NB>
 static size_t step_lds_bytes(int k) {
-    return sizeof(double) * ((size_t)NB * (NB + 1) + (size_t)k * (NB + 1));
+    // Ld, then the panel P (k x (NB + 1)), whose space also holds the
+    // per-wave partial sums (ST / 64) x k of the final row passes
+    const size_t pn = (size_t)k * (NB + 1) > (size_t)(ST / 64) * k ? (size_t)k * (NB + 1)
+                                                                    : (size_t)(ST / 64) * k;
+    return sizeof(double) * ((size_t)NB * (NB + 1) + pn);
 }
 
 template <int NB>

@@ -36,6 +36,33 @@ def inv_softplus(y):
     return y + np.log(-np.expm1(-y))
 
 
+class NLMLFunction(torch.autograd.Function):
+    """NLML as a torch.autograd.Function whose backward is the hand-written
+    adjoint (replacing TF autodiff through GPflow's training_loss,
+    main.py:89-91).  apply(x, engine, unconstrained): x = (nu, kappa,
+    sigma_f, noise_variance) -- or, with unconstrained=True, the
+    softplus-unconstrained u -- as a (4,) FP64 device tensor; returns the
+    scalar NLML.  forward runs the engine's evaluate() (NLML and gradient in
+    one pass); backward only scales the cached gradient."""
+
+    @staticmethod
+    def forward(ctx, x, engine, unconstrained=False):
+        xd = x.detach().to(engine.dev, torch.float64)
+        if unconstrained:
+            engine.u.copy_(xd)
+        else:
+            y = xd - engine.lower
+            engine.u.copy_(y + torch.log(-torch.expm1(-y)))  # inverse softplus
+        out = engine.evaluate()
+        ctx.save_for_backward((out[5:9] if unconstrained else out[1:5]).clone())
+        return out[0].clone()
+
+    @staticmethod
+    def backward(ctx, g):
+        (grad,) = ctx.saved_tensors
+        return g * grad, None, None
+
+
 class GPREngine:
     """Hyperparameters theta = (nu, kappa, sigma_f, s2) = softplus(u) + lower."""
 
@@ -211,6 +238,10 @@ class GPREngine:
         u = dense.gemm(self.Linv, t, transA=True)
         return (r - dense.gemm(self.Phi, u)) / self.s2
 
+    def nlml(self, x, unconstrained=False):
+        """Differentiable NLML at x (see NLMLFunction)."""
+        return NLMLFunction.apply(x, self, unconstrained)
+
     # ---------------------------------------------------------------- host API
     def nlml_grad_u(self, u):
         self.u.copy_(torch.as_tensor(np.asarray(u, np.float64), device=self.dev))
@@ -267,6 +298,7 @@ class GPREngine:
         res = scipy.optimize.minimize(f, u0, jac=True, method="L-BFGS-B",
                                       options={"disp": disp, "maxiter": maxiter})
         self.u.copy_(torch.as_tensor(res.x, device=self.dev))
+        self.last_nfev = int(res.nfev)
         return res
 
     def adam_setup(self, trainable=(1, 1, 1, 1)):

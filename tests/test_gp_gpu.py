@@ -200,3 +200,24 @@ def test_multi_rhs_vs_oracle(form, precision):
         mc = m[:, r].cpu().numpy()
         assert np.linalg.norm(mc - mr.ravel()) / np.linalg.norm(mr) < 1e-6
     np.testing.assert_allclose(v.cpu().numpy(), vr.ravel(), rtol=1e-6, atol=1e-10)
+
+
+@pytest.mark.parametrize("form", ["function", "weight"])
+def test_autograd_function_uses_the_adjoint(form):
+    """NLMLFunction: the value is the engine's NLML, torch.autograd returns
+    the adjoint gradient (constrained and unconstrained parameters), and the
+    backward passes torch's own finite-difference gradcheck."""
+    import torch
+    Phi, y, lam = problem(M_nodes=80, k=20, seed=13)
+    th = (1.4, 3.5, 0.9, 0.2)
+    e = engine(Phi, y, lam, 700.0, form, theta=th)
+    ref = e.evaluate().cpu().numpy().copy()
+    x = torch.tensor(th, dtype=torch.float64, device="cuda", requires_grad=True)
+    val = e.nlml(x)
+    (g,) = torch.autograd.grad(val, x)
+    assert abs(val.item() - ref[0]) < 1e-12 * abs(ref[0])
+    np.testing.assert_allclose(g.cpu().numpy(), ref[1:5], rtol=1e-12)
+    u = e.u.clone().requires_grad_(True)
+    (gu,) = torch.autograd.grad(e.nlml(u, unconstrained=True), u)
+    np.testing.assert_allclose(gu.cpu().numpy(), ref[5:9], rtol=1e-12)
+    assert torch.autograd.gradcheck(lambda t: e.nlml(t), (x,), eps=1e-6, atol=1e-4, rtol=1e-4)

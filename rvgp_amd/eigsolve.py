@@ -147,13 +147,16 @@ def _deflate(Y, Xl):
 
 
 def _run_phase(A, X, k, ub, wdt, ptol, max_iter, max_ratio, lock, fresh, stats, verbose, tag,
-               stall_exit=False, m_max=60, ritz=None):
+               stall_exit=False, m_max=60, ritz=None, scale=None):
     """Filter + Rayleigh-Ritz iterations on block X (work dtype wdt) until the
     k wanted pairs have residual/ub < ptol.  With ``lock``, converged leading
     pairs are frozen and deflated out of the filtered block.  ``ritz`` =
     (theta, worst residual / ub) of X when X already is a Ritz basis (the
     previous phase's result): the phase then opens with a filter pass instead
-    of a Rayleigh-Ritz step, which would only re-derive those values."""
+    of a Rayleigh-Ritz step, which would only re-derive those values.
+    ``ub`` bounds the spectrum for the filters; residuals are measured
+    relative to ``scale`` (default ub)."""
+    scale = ub if scale is None else scale
     p = X.shape[1]
     history = []
     Xl = X[:, :0]
@@ -181,7 +184,7 @@ def _run_phase(A, X, k, ub, wdt, ptol, max_iter, max_ratio, lock, fresh, stats, 
         stats["rr"] += 1
         stats["spmm"] += 1
         stats["t_rr"] += time.perf_counter() - tr
-        res = _residuals(Xa, tha, AXa) / ub
+        res = _residuals(Xa, tha, AXa) / scale
         need = k - Xl.shape[1]
         kk = min(need, res.numel())
         worst = float(res[:kk].max().item()) if kk > 0 else 0.0
@@ -263,7 +266,12 @@ def chfsi(A: BSR, k, precision="fp64", tol=1e-9, max_iter=300, guard=None, seed=
     if guard is None:
         guard = max(8, int(math.ceil(0.15 * k)))
     p = min(N, -(-(k + guard) // 8) * 8)
-    ub = A.gershgorin_bound() * 1.0001 + 1e-12
+    # residuals are measured against the Gershgorin bound (a norm scale that
+    # does not depend on a random start); the filters use the tighter
+    # Lanczos bound of lambda_max
+    scale = A.gershgorin_bound() * 1.0001 + 1e-12
+    ub = min(scale, A.spectral_upper_bound() * 1.0001 + 1e-12)
+    stats_ub = ub
     stats = EigStats(spmm=0, rr=0, iters=0, t_filter=0.0, t_rr=0.0)
     gen = torch.Generator(device=dev)
     gen.manual_seed(int(seed))
@@ -279,32 +287,35 @@ def chfsi(A: BSR, k, precision="fp64", tol=1e-9, max_iter=300, guard=None, seed=
         # stalls there)
         X, theta, it, _ = _run_phase(A, X.to(torch.float32).contiguous(), k, ub, torch.float32,
                                      1e-5, max_iter, 300.0, True, fresh, stats, verbose, "fp32",
-                                     stall_exit=True)
+                                     stall_exit=True, scale=scale)
         stats["iters_fp32"] = it
         stats["t_fp32"] = time.perf_counter() - t0
         # FP64 polish with locking: the FP32 block is already accurate to
         # ~1e-5, so a few filter passes finish the pairs
         X, theta, it, done = _run_phase(A, X.double().contiguous(), k, ub, torch.float64, tol, 50,
                                         1e4, True, fresh, stats, verbose, "fp64-polish",
-                                        ritz=(theta, stats["last_worst"]))
+                                        ritz=(theta, stats["last_worst"]), scale=scale)
         if not done:
             # an unlocked Rayleigh-Ritz over the whole block orders and refines
             X, theta, _, _ = _run_phase(A, X, k, ub, torch.float64, tol, 3, 1e4, False, fresh,
-                                        stats, verbose, "fp64-final", m_max=16)
+                                        stats, verbose, "fp64-final", m_max=16, scale=scale)
         stats["iters_polish"] = it
     elif precision == "fp64":
         X, theta, it, done = _run_phase(A, X.to(torch.float64).contiguous(), k, ub, torch.float64,
-                                        tol, max_iter, 1e4, True, fresh, stats, verbose, "fp64")
+                                        tol, max_iter, 1e4, True, fresh, stats, verbose, "fp64",
+                                        scale=scale)
         if not done:
             X, theta, _, _ = _run_phase(A, X, k, ub, torch.float64, tol, 3, 1e4, False, fresh,
-                                        stats, verbose, "fp64-final", m_max=16)
+                                        stats, verbose, "fp64-final", m_max=16, scale=scale)
     else:
         raise ValueError("precision must be 'fp64' or 'mixed'")
     stats["t_total"] = time.perf_counter() - t0
     o = torch.argsort(theta)
     theta, X = theta[o], X[:, o]
     AX = spmm(A, X[:, :k].contiguous())
-    stats["residual"] = float(_residuals(X[:, :k], theta[:k], AX).max().item() / ub)
+    stats["residual"] = float(_residuals(X[:, :k], theta[:k], AX).max().item() / scale)
+    stats["ub_filter"] = stats_ub
+    stats["ub_gershgorin"] = scale
     evals = theta[:k].clone()
     evecs = X[:, :k].double()
     evecs = evecs / torch.sqrt(dense.colsumsq(evecs))

@@ -3,7 +3,8 @@
 The reference forms exp(-t L) densely with scipy.linalg.expm
 (smoothing.py:13-15), O((n d)^3) time and O((n d)^2) memory.  Here
 exp(-t A) X is a Chebyshev expansion on the spectral interval [0, ub]
-(ub >= lambda_max by Gershgorin):
+(ub >= lambda_max: the Lanczos bound of BSR.spectral_upper_bound, at most
+the Gershgorin bound):
 
     exp(-t lambda) = sum_j c_j T_j(2 lambda / ub - 1),
     c_j = (2 - [j = 0]) (-1)^j I_j(t ub / 2) exp(-t ub / 2),
@@ -42,7 +43,7 @@ def expm_multiply(A: BSR, X, t, ub=None):
     then runs once, to the longest series, and each column accumulates its
     own coefficients."""
     if ub is None:
-        ub = A.gershgorin_bound() * 1.0001 + 1e-12
+        ub = A.spectral_upper_bound() * 1.0001 + 1e-12
     X2 = X.reshape(-1, 1) if X.dim() == 1 else X
     ts = np.atleast_1d(np.asarray(t, dtype=np.float64))
     if ts.size > 1:

@@ -210,3 +210,24 @@ def test_vector_diffusion_per_field_times_vs_oracle():
         raw = cpu.heat(Lc, t, cl[f].reshape(-1, 1), "eigh")
         amp = np.linalg.norm(cl[f]) / np.linalg.norm(raw)
         assert np.abs(out[f] - ref).max() < 1e-12 * amp * np.abs(ref).max(), (f, t)
+
+
+def test_spectral_upper_bound_brackets_lambda_max():
+    """The Lanczos bound used by the filters and the heat-kernel series lies
+    between lambda_max (dense eigh) and the Gershgorin bound, and within 3 %
+    of lambda_max (2 % margin)."""
+    X = sphere(400, 11)
+    rp, c = cpu.sym_csr(cpu.knn(X, 10))
+    T, S = cpu.tangent_frames(X, rp, c, 3, 15)
+    T2 = np.ascontiguousarray(T[:, :, :2])
+    R = cpu.connections(T2, rp, c)
+    L = cpu.laplacian(rp, c)
+    rows = np.repeat(np.arange(400), np.diff(rp))
+    for A, dense_A in ((_bsr_from_blocks(rp, c, cpu.lc_blocks(rp, c, R), 2),
+                        cpu.connection_laplacian(rp, c, R).toarray()),
+                       (_bsr_from_blocks(rp, c, np.asarray(L[rows, c]).ravel(), 1),
+                        L.toarray())):
+        lmax = np.linalg.eigvalsh(dense_A)[-1]
+        ub = A.spectral_upper_bound()
+        assert lmax <= ub <= A.gershgorin_bound() + 1e-12
+        assert ub <= 1.03 * lmax, (ub, lmax)

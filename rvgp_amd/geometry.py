@@ -101,11 +101,26 @@ def manifold_graph(X, typ='knn', n_neighbors=5):
 # ---------------------------------------------------------------------------
 # a6 / a7  geometry.py:14-63  (host scipy views of the device operators)
 # ---------------------------------------------------------------------------
+def _networkx_degrees(G):
+    """Node degrees as networkx's G.degree() reports them: unweighted, a self
+    loop counted twice."""
+    if not isinstance(G, SymGraph):
+        return np.array(list(dict(G.degree()).values()), dtype=np.float64)
+    rp, c = G.csr_numpy()
+    rows = np.repeat(np.arange(G.n), np.diff(rp))
+    loops = np.bincount(rows[c == rows], minlength=G.n)
+    return (np.diff(rp) + loops).astype(np.float64)
+
+
 def compute_laplacian(G, normalization=False):
-    """L = D - A of the symmetrised graph, as scipy csr (geometry.py:55-63)."""
-    if normalization:
-        raise NotImplementedError("normalised Laplacian is not on the RVGP path")
+    """L = D - A of the symmetrised graph, as scipy csr (geometry.py:55-63);
+    normalization=True gives networkx's normalized_laplacian_matrix of the
+    same graph (the reference's host call, geometry.py:58-59)."""
     import scipy.sparse as sp
+    if normalization:
+        import networkx as nx
+        Gx = G.to_networkx() if isinstance(G, SymGraph) else G
+        return sp.csr_matrix(nx.normalized_laplacian_matrix(Gx), dtype=np.float64)
     g = as_device_graph(G)
     A = g.to_scipy()
     deg = np.asarray(A.sum(1)).ravel()
@@ -120,7 +135,9 @@ def compute_connection_laplacian(G, R, normalization=None):
     L = compute_laplacian(G)
     Lc = sp.csr_matrix(sp.kron(L, np.ones([dim, dim])).multiply(R))
     if normalization == "rw":
-        deg = np.diff(as_device_graph(G).rowptr.cpu().numpy()) - 1.0
+        # D^-1 Lc with networkx's degrees (geometry.py:45-50): the kNN + I
+        # graph's self loop counts twice, so deg_i = neighbours + 2
+        deg = _networkx_degrees(G)
         deg_inv = np.where(deg > 0, 1.0 / np.where(deg > 0, deg, 1.0), 0.0).repeat(dim)
         Lc = sp.diags(deg_inv, 0, format='csr') @ Lc
     return Lc

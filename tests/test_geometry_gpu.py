@@ -246,3 +246,32 @@ def test_bsr_spmm(b, r):
         Y2 = spmm(A, Xd, Y=Zd, alpha=1.0, Z=Zd, gamma=0.0)
         assert np.abs(Y2.cpu().double().numpy() - As @ X).max() < tol * np.abs(As @ X).max() * 10
 
+
+
+def test_rw_connection_laplacian_uses_networkx_degrees():
+    """compute_connection_laplacian(G, R, 'rw') (geometry.py:45-50): D^-1 Lc
+    with networkx's degrees -- the kNN + I graph's self loop counts twice --
+    for both a networkx graph and the device graph; compute_laplacian(G,
+    True) is networkx's normalised Laplacian."""
+    import networkx as nx
+    import scipy.sparse as sp
+    from sklearn.neighbors import kneighbors_graph
+    from rvgp_amd import geometry
+    X = sphere(150, 21)
+    A = kneighbors_graph(X, 8, mode='connectivity', include_self=False) + sp.eye(150)
+    Gx = nx.from_scipy_sparse_array(A)
+    Gd = geometry.manifold_graph(X, n_neighbors=8)
+    d = 2
+    rng = np.random.default_rng(2)
+    pat = sp.kron(nx.adjacency_matrix(Gx), np.ones((d, d))).tocsr()
+    R = pat.multiply(rng.normal(size=pat.shape)).tocsr()
+    Lref = sp.kron(sp.csr_matrix(nx.laplacian_matrix(Gx), dtype=np.float64), np.ones((d, d)))
+    Lref = Lref.multiply(R)
+    deg = np.array(list(dict(Gx.degree()).values()), dtype=np.float64)
+    Lref = (sp.diags((1.0 / deg).repeat(d)) @ Lref).toarray()
+    for G in (Gx, Gd):
+        out = geometry.compute_connection_laplacian(G, R, normalization="rw").toarray()
+        np.testing.assert_allclose(out, Lref, rtol=1e-15, atol=1e-15)
+    Nref = nx.normalized_laplacian_matrix(Gx).toarray()
+    for G in (Gx, Gd):
+        np.testing.assert_allclose(geometry.compute_laplacian(G, True).toarray(), Nref, atol=1e-15)

@@ -279,3 +279,27 @@ def test_train_test_split_rule_matches_sklearn():
         tr2, te2 = cpu.train_test_split_indices(n, 0.2, seed)
         np.testing.assert_array_equal(tr, tr2)
         np.testing.assert_array_equal(te, te2)
+
+
+def test_load_mesh_reads_the_reference_cube(golden_dir, tmp_path):
+    """utils.load_mesh on the reference's cube.obj (Blender export: '#'
+    comments, 'o' and 's' records, 8 vertices, 12 one-based faces)."""
+    from rvgp_amd.utils import load_mesh
+    v, f = load_mesh('cube', folder=golden_dir)
+    assert v.shape == (8, 3) and v.dtype == np.float64
+    assert f.shape == (12, 3)
+    np.testing.assert_array_equal(v[0], [1.0, 1.0, -1.0])
+    np.testing.assert_array_equal(v[7], [-1.0, -1.0, 1.0])
+    np.testing.assert_array_equal(f[0], [4, 2, 0])
+    np.testing.assert_array_equal(f[-1], [4, 0, 1])
+    assert f.min() == 0 and f.max() == 7
+    # every vertex of a closed triangulated cube is used; 12 faces x 3 = 36
+    assert np.bincount(f.ravel()).sum() == 36
+    # reference error behaviour: a blank line -> IndexError, 'f 1/1 2/2 3/3' -> ValueError
+    import pytest
+    (tmp_path / "blank.obj").write_text("v 0 0 0\n\nv 1 1 1\n")
+    with pytest.raises(IndexError):
+        load_mesh('blank', folder=str(tmp_path))
+    (tmp_path / "tex.obj").write_text("v 0 0 0\nf 1/1 2/2 3/3\n")
+    with pytest.raises(ValueError):
+        load_mesh('tex', folder=str(tmp_path))

@@ -322,6 +322,31 @@ int rvgp_batch_step(int F, int k, int typ, const double* evals, double n_rows, c
                     int do_adam, double lr, double b1, double b2, double eps, int panel, int grid,
                     double* ws, size_t ws_bytes, double* out, int* status, void* stream);
 
+/* ---------------------------------------------------------------------
+ * kernel='rbf' branch of RVGP.fit (RVGP/main.py:33-37): gpflow.kernels.RBF
+ * on evecs_L rows, the D vector components as D output columns of one GPR.
+ * theta = (variance, lengthscale, noise) = softplus(u) + lower.
+ * ------------------------------------------------------------------- */
+int rvgp_rbf_params(const double* u, const double* lower, double* theta, void* stream);
+
+/* K (M x N) = variance exp(-r^2 / (2 lengthscale^2)), r^2 = max(0, rn_i + cn_j
+ * - 2 G_ij) from the Gram G = X X2^T and squared row norms (GPflow's
+ * square_distance); sym: X2 = X (r_ii = 0); add_noise: + noise on the
+ * diagonal. */
+int rvgp_rbf_kernel(int M, int N, const double* G, int ldg, const double* rn, const double* cn,
+                    const double* theta, int sym, int add_noise, double* K, int ldk,
+                    void* stream);
+
+/* Summed NLML over nrhs outputs and its adjoint, out (7) = [nlml,
+ * d/d(variance, lengthscale, noise), d/du(...)], from Kinv = K_y^-1, AAt =
+ * A A^T (A = K_y^-1 Y), the Gram G, squared norms sqn, ya = sum y^T K_y^-1 y
+ * and logdet_half = sum log L_ii.  ws: rvgp_rbf_grad_ws_bytes(M) bytes. */
+size_t rvgp_rbf_grad_ws_bytes(int M);
+int rvgp_rbf_grad(int M, int nrhs, const double* Kinv, int ldk, const double* AAt, int lda,
+                  const double* G, int ldg, const double* sqn, const double* theta, const double* ya,
+                  const double* logdet_half, const double* u, double* ws, size_t ws_bytes,
+                  double* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -443,6 +443,51 @@ def gpr_predict(Phi, y, S, noise, Phi_star):
 
 
 # --------------------------------------------------------------------------
+# kernel='rbf' branch (main.py:33-37): gpflow.kernels.RBF on evecs_L rows,
+# the D vector components as D output columns of one GPR
+# --------------------------------------------------------------------------
+def rbf_kernel(X, X2, variance, lengthscale):
+    """variance * exp(-|x - x'|^2 / (2 lengthscale^2)) (squared distances by
+    scipy.spatial.distance.cdist)."""
+    from scipy.spatial.distance import cdist
+    return variance * np.exp(-0.5 * cdist(X, X2, "sqeuclidean") / lengthscale ** 2)
+
+
+def rbf_gpr_nlml(X, Y, variance, lengthscale, noise):
+    """Summed NLML of the R = Y.shape[1] output columns."""
+    M, R = Y.shape
+    L = np.linalg.cholesky(rbf_kernel(X, X, variance, lengthscale) + noise * np.eye(M))
+    a = scipy.linalg.solve_triangular(L, Y, lower=True)
+    return (0.5 * float((a * a).sum()) + R * float(np.log(np.diag(L)).sum())
+            + 0.5 * R * M * np.log(2 * np.pi))
+
+
+def rbf_gpr_grad(X, Y, variance, lengthscale, noise):
+    """d NLML / d(variance, lengthscale, noise) = 1/2 tr((R K^-1 - A A^T) dK),
+    A = K^-1 Y."""
+    from scipy.spatial.distance import cdist
+    M, R = Y.shape
+    D2 = cdist(X, X, "sqeuclidean")
+    Kf = variance * np.exp(-0.5 * D2 / lengthscale ** 2)
+    Kinv = np.linalg.inv(Kf + noise * np.eye(M))
+    A = Kinv @ Y
+    W = R * Kinv - A @ A.T
+    return np.array([0.5 * (W * Kf).sum() / variance,
+                     0.5 * (W * Kf * D2).sum() / lengthscale ** 3,
+                     0.5 * np.trace(W)])
+
+
+def rbf_gpr_predict(X, Y, variance, lengthscale, noise, Xs):
+    """Mean (m*, R) and marginal variance (m*,)."""
+    M = X.shape[0]
+    L = np.linalg.cholesky(rbf_kernel(X, X, variance, lengthscale) + noise * np.eye(M))
+    Ks = rbf_kernel(X, Xs, variance, lengthscale)
+    V = scipy.linalg.solve_triangular(L, Ks, lower=True)
+    a = scipy.linalg.solve_triangular(L, Y, lower=True)
+    return V.T @ a, variance - (V * V).sum(0)
+
+
+# --------------------------------------------------------------------------
 # a13 / a15  fit driver  (main.py:11-95)
 # --------------------------------------------------------------------------
 def train_test_split_indices(n, test_size=0.2, seed=0):

@@ -59,6 +59,10 @@ SIGNATURES = {
     "rvgp_batch_gram": (I, [I, I, I, I, P, I, P, P, I, P, P]),
     "rvgp_batch_rhs": (I, [I, I, I, P, I, P, P, P, P, P, P, P]),
     "rvgp_batch_step_ws_bytes": (S, [I, I]),
+    "rvgp_rbf_params": (I, [P, P, P, P]),
+    "rvgp_rbf_kernel": (I, [I, I, P, I, P, P, P, I, I, P, I, P]),
+    "rvgp_rbf_grad_ws_bytes": (S, [I]),
+    "rvgp_rbf_grad": (I, [I, I, P, I, P, I, P, I, P, P, P, P, P, P, S, P, P]),
     "rvgp_batch_step": (I, [I, I, I, P, D, P, P, P, P, P, P, P, P, P, P, I, D, D, D, D, I, I, P, S,
                             P, P, P]),
 }

@@ -102,3 +102,13 @@ class ManifoldKernel:
 
     def __call__(self, X, X2=None):
         return self.K(X, X2)
+
+
+class RBF:
+    """gpflow.kernels.RBF as the reference creates it for kernel='rbf'
+    (main.py:33-37): variance and lengthscales start at 1.0, positive with
+    the default minimum in force when the kernel is created."""
+
+    def __init__(self, variance=1.0, lengthscales=1.0):
+        self.variance = Parameter(variance, name='variance')
+        self.lengthscales = Parameter(lengthscales, name='lengthscales')

@@ -19,7 +19,7 @@ import torch
 
 from . import _lib
 from .gp import GPREngine
-from .kernels import ManifoldKernel, set_default_positive_minimum
+from .kernels import RBF, ManifoldKernel, set_default_positive_minimum
 
 GAUSSIAN_VARIANCE_LOWER_BOUND = 1e-6  # gpflow.likelihoods.Gaussian default
 
@@ -103,8 +103,10 @@ def train_gp(data, train_ind=None, n_inducing_points=None, test_size=0.2, kernel
         train_ind = np.nonzero(train_ind)[0]
     if kernel is None:
         kernel = ManifoldKernel(data, nu=3 / 2, kappa=5, typ='matern', sigma_f=1.)
-    elif kernel == 'rbf':
-        raise NotImplementedError("kernel='rbf' (main.py:33-37) is not on the GPU path")
+    elif isinstance(kernel, str) and kernel == 'rbf':
+        kernel = RBF()
+        if verbose:
+            print('Using RBF kernel, treating vectors channel-wise.')
     else:
         raise UnboundLocalError("local variable 'input' referenced before assignment")
 
@@ -118,6 +120,9 @@ def train_gp(data, train_ind=None, n_inducing_points=None, test_size=0.2, kernel
     if n_inducing_points is not None:
         raise NotImplementedError("SGPR inducing points (main.py:59-67) are not on the GPU path")
 
+    if isinstance(kernel, RBF):
+        return _train_rbf(data, kernel, nodes_tr, nodes_te, noise_variance, kernel_lengthscale,
+                          kernel_variance, epochs, optimizer, reference_semantics, verbose, timings)
     if kernel_variance is not None:
         raise AttributeError("'ManifoldKernel' object has no attribute 'variance'")
     if kernel_lengthscale is not None:
@@ -175,6 +180,108 @@ def train_gp(data, train_ind=None, n_inducing_points=None, test_size=0.2, kernel
         model.heldout_error = err
         if verbose:
             print("Relative l2 error is {}".format(err))
+    clock.tick("gp_heldout_predict")
+    return model
+
+
+class rbf_GPR:
+    """The fitted kernel='rbf' model: one GPR over the D vector components
+    as output columns, inputs = rows of evecs_L (main.py:33-37, 98-116)."""
+
+    def __init__(self, data, kernel, train_nodes, engine, reference_semantics=True):
+        self.data = data
+        self.kernel = kernel
+        self.train_nodes = train_nodes
+        self.engine = engine
+        self.reference_semantics = reference_semantics
+
+    @property
+    def hyperparameters(self):
+        th = self.engine.theta_host()
+        return {"variance": th[0], "lengthscales": th[1], "noise_variance": th[2]}
+
+    def training_loss(self):
+        return self.engine.nlml_grad_u(self.engine.u.cpu().numpy())[0]
+
+    def predict_f(self, Xnew):
+        dev = self.engine.dev
+        Xd = Xnew if isinstance(Xnew, torch.Tensor) else torch.as_tensor(
+            np.asarray(Xnew, np.float64), device=dev)
+        if Xd.shape[1] != self.engine.k:
+            raise ValueError(f"RBF inputs have {Xd.shape[1]} columns; the model was trained on "
+                             f"{self.engine.k} (rows of evecs_L)")
+        mean, var = self.engine.predict(Xd)
+        return mean, var.reshape(-1, 1).expand(-1, self.engine.R)
+
+    def transform(self, data, test_ind):
+        """main.py:102-116.  With reference_semantics the reference's int
+        branch is kept as it is: it reads rows of evecs_Lc (not evecs_L)
+        reshaped to (len * D, k), which only runs when both spectra have the
+        same number of eigenpairs, and returns (len, D * D) arrays.  Without,
+        the test nodes' evecs_L rows are used: (len, D) mean and variance."""
+        first = test_ind[0]
+        ints = (int,) if self.reference_semantics else (int, np.integer)
+        floats = (float,) if self.reference_semantics else (float, np.floating)
+        if isinstance(first, ints):
+            idx = torch.as_tensor(np.asarray(test_ind).astype(np.int64), device=self.engine.dev)
+            if self.reference_semantics:
+                E = torch.as_tensor(np.asarray(data.evecs_Lc), device=self.engine.dev)
+                test_x = E.reshape(data.n, -1)[idx].reshape(-1, E.shape[1])
+            else:
+                test_x = data._evecs_L[idx]
+        elif isinstance(first, floats):
+            test_x = np.asarray(test_ind, dtype=np.float64).reshape(-1, self.engine.k)
+        else:
+            raise UnboundLocalError("local variable 'test_x' referenced before assignment")
+        mean, var = self.predict_f(test_x)
+        return (mean.cpu().numpy().reshape(len(test_ind), -1),
+                var.cpu().numpy().reshape(len(test_ind), -1))
+
+
+def _train_rbf(data, kernel, nodes_tr, nodes_te, noise_variance, kernel_lengthscale,
+               kernel_variance, epochs, optimizer, reference_semantics, verbose, timings):
+    """kernel='rbf' (main.py:33-37): inputs = evecs_L rows of the training
+    nodes, outputs = their D vector components as D columns; GPflow's GPR
+    with the Gaussian noise starting at 1.0 under reference semantics;
+    kernel_variance / kernel_lengthscale fix those parameters
+    (main.py:69-75); L-BFGS-B with maxiter=epochs (main.py:87-95)."""
+    from .rbf import RBFEngine
+    if optimizer != "lbfgs":
+        raise NotImplementedError("kernel='rbf' is fitted with L-BFGS-B, as in the reference")
+    dev = _lib.device()
+    clock = _Clock(timings)
+    V = data.vectors_device()
+    if V is None:
+        raise TypeError("data.vectors is None: nothing to fit")
+    if V.dim() != 2:
+        raise ValueError("kernel='rbf' fits one field (vectors of shape (n, D))")
+    idx = torch.as_tensor(nodes_tr, device=dev).long()
+    X = data._evecs_L[idx].contiguous()
+    Y = V[idx].contiguous()
+    noise0 = 1.0 if reference_semantics else float(noise_variance)
+    trainable = [True, True, True]
+    if kernel_variance is not None:
+        kernel.variance.assign(kernel_variance)
+        trainable[0] = False
+    if kernel_lengthscale is not None:
+        kernel.lengthscales.assign(kernel_lengthscale)
+        trainable[1] = False
+    eng = RBFEngine(X, Y, lower=(kernel.variance.lower, kernel.lengthscales.lower,
+                                 GAUSSIAN_VARIANCE_LOWER_BOUND), trainable=trainable)
+    eng.set_theta([float(kernel.variance), float(kernel.lengthscales), noise0])
+    clock.tick("gp_setup")
+    eng.fit_lbfgs(maxiter=epochs, disp=verbose)
+    clock.tick("gp_optimise")
+    th = eng.theta_host()
+    kernel.variance.assign(th[0])
+    kernel.lengthscales.assign(th[1])
+    model = rbf_GPR(data, kernel, nodes_tr, eng, reference_semantics)
+    if len(nodes_te):
+        mean, _ = model.predict_f(data._evecs_L[torch.as_tensor(nodes_te, device=dev).long()])
+        out_test = V[torch.as_tensor(nodes_te, device=dev).long()]
+        model.heldout_error = torch.linalg.vector_norm(out_test - mean, dim=1).mean().item()
+        if verbose:
+            print("Relative l2 error is {}".format(model.heldout_error))
     clock.tick("gp_heldout_predict")
     return model
 

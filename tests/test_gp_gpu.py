@@ -221,3 +221,66 @@ def test_autograd_function_uses_the_adjoint(form):
     (gu,) = torch.autograd.grad(e.nlml(u, unconstrained=True), u)
     np.testing.assert_allclose(gu.cpu().numpy(), ref[5:9], rtol=1e-12)
     assert torch.autograd.gradcheck(lambda t: e.nlml(t), (x,), eps=1e-6, atol=1e-4, rtol=1e-4)
+
+
+@pytest.mark.parametrize("M,R", [(60, 3), (300, 2)])
+def test_rbf_engine_vs_oracle(M, R):
+    """kernel='rbf' engine: summed NLML over the output columns, the
+    adjoint w.r.t. (variance, lengthscale, noise) and the predictive mean /
+    variance vs the textbook RBF GPR of the oracle."""
+    import torch
+    from rvgp_amd.rbf import RBFEngine
+    rng = np.random.default_rng(M)
+    X = rng.normal(size=(M, 12))
+    Y = rng.normal(size=(M, R))
+    th = (1.4, 2.3, 0.07)
+    e = RBFEngine(torch.as_tensor(X, device="cuda"), torch.as_tensor(Y, device="cuda"))
+    e.set_theta(th)
+    out = e.evaluate().cpu().numpy()
+    ref = cpu.rbf_gpr_nlml(X, Y, *th)
+    g = cpu.rbf_gpr_grad(X, Y, *th)
+    assert abs(out[0] - ref) < 1e-10 * abs(ref)
+    np.testing.assert_allclose(out[1:4], g, rtol=1e-8, atol=1e-10 * np.abs(g).max())
+    Xs = rng.normal(size=(25, 12))
+    m, v = e.predict(torch.as_tensor(Xs, device="cuda"))
+    mr, vr = cpu.rbf_gpr_predict(X, Y, *th, Xs)
+    np.testing.assert_allclose(m.cpu().numpy(), mr, rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose(v.cpu().numpy(), vr, rtol=1e-9, atol=1e-12)
+
+
+def test_fit_rbf_kernel_vs_oracle_lbfgs():
+    """RVGP.fit(d, kernel='rbf') end to end: evecs_L rows in, the D
+    components as output columns, L-BFGS-B from (1, 1, noise 1.0): the
+    optimum matches the oracle's L-BFGS-B on the same rows, and a fixed
+    kernel_variance stays fixed."""
+    import rvgp_amd as RVGP
+    from tests.util import sphere
+    X = sphere(400, 31)
+    d = RVGP.create_data_object(X, n_neighbors=10, n_eigenpairs=20, verbose=False)
+    d.random_vector_field(seed=2)
+    d.smooth_vector_field(t=10)
+    train = np.random.default_rng(4).choice(400, size=250, replace=False)
+    gp = RVGP.fit(d, train_ind=train, kernel='rbf', epochs=200, verbose=False)
+    tr, te = cpu.train_test_split_indices(len(train), 0.2, 0)
+    nodes = train[tr]
+    Xtr = np.asarray(d.evecs_L)[nodes]
+    Ytr = np.asarray(d.vectors)[nodes]
+    import scipy.optimize
+
+    def f(u):
+        th = cpu.softplus(u) + np.array([0.0, 0.0, 1e-6])
+        return (cpu.rbf_gpr_nlml(Xtr, Ytr, *th),
+                cpu.rbf_gpr_grad(Xtr, Ytr, *th) / (1.0 + np.exp(-u)))
+    u0 = cpu.inv_softplus(np.array([1.0, 1.0, 1.0 - 1e-6]))
+    res = scipy.optimize.minimize(f, u0, jac=True, method="L-BFGS-B", options={"maxiter": 200})
+    th_ref = cpu.softplus(res.x) + np.array([0.0, 0.0, 1e-6])
+    th = gp.engine.theta_host()
+    np.testing.assert_allclose(th, th_ref, rtol=2e-3)
+    assert abs(gp.training_loss() - res.fun) < 1e-6 * abs(res.fun)
+    # held-out error on the re-split nodes, from the fitted model
+    Xte = np.asarray(d.evecs_L)[train[te]]
+    m, _ = cpu.rbf_gpr_predict(Xtr, Ytr, *th, Xte)
+    err = np.linalg.norm(np.asarray(d.vectors)[train[te]] - m, axis=1).mean()
+    assert abs(gp.heldout_error - err) < 1e-8
+    gp2 = RVGP.fit(d, train_ind=train, kernel='rbf', kernel_variance=0.5, epochs=50, verbose=False)
+    assert gp2.hyperparameters["variance"] == pytest.approx(0.5, rel=1e-12)

@@ -303,3 +303,27 @@ def test_load_mesh_reads_the_reference_cube(golden_dir, tmp_path):
     (tmp_path / "tex.obj").write_text("v 0 0 0\nf 1/1 2/2 3/3\n")
     with pytest.raises(ValueError):
         load_mesh('tex', folder=str(tmp_path))
+
+
+def test_rbf_gpr_oracle_vs_multivariate_normal_and_fd():
+    """The RBF-branch oracle: summed NLML = sum of scipy log-densities,
+    gradient = central differences, noise-free prediction interpolates."""
+    from scipy.stats import multivariate_normal
+    rng = np.random.default_rng(17)
+    X = rng.normal(size=(40, 6))
+    Y = rng.normal(size=(40, 3))
+    th = (1.3, 2.1, 0.05)
+    K = cpu.rbf_kernel(X, X, th[0], th[1]) + th[2] * np.eye(40)
+    ref = -sum(multivariate_normal(mean=np.zeros(40), cov=K).logpdf(Y[:, r]) for r in range(3))
+    assert abs(cpu.rbf_gpr_nlml(X, Y, *th) - ref) < 1e-10 * abs(ref)
+    g = cpu.rbf_gpr_grad(X, Y, *th)
+    h = 1e-6
+    for p in range(3):
+        tp, tm = list(th), list(th)
+        tp[p] += h
+        tm[p] -= h
+        fd = (cpu.rbf_gpr_nlml(X, Y, *tp) - cpu.rbf_gpr_nlml(X, Y, *tm)) / (2 * h)
+        assert abs(fd - g[p]) < 1e-5 * max(1.0, abs(fd)), (p, fd, g[p])
+    m, v = cpu.rbf_gpr_predict(X, Y, 1.0, 1.5, 1e-10, X)
+    np.testing.assert_allclose(m, Y, atol=1e-5)
+    assert np.abs(v).max() < 1e-5

@@ -136,6 +136,28 @@ def _degree(theta, k, cut, ub, max_ratio, m_min=4, m_max=60, need=None):
     return max(m_min, min(m_max, m)), m_need
 
 
+def _random_block_bracket(A, p):
+    """Where the Ritz values of a random p-column block of A lie, without
+    forming it: around the mean eigenvalue mu = tr(A) / N, within
+    +-2 sigma sqrt(p / N), sigma^2 = |A|_F^2 / N - mu^2 the eigenvalue variance
+    (the edge of the Marchenko-Pastur law for a random projection).  The
+    opening filter of a random block takes its bounds and degree from this
+    bracket in place of a Rayleigh-Ritz step on the unfiltered block."""
+    N = A.shape[0]
+    b = A.b
+    rows = torch.repeat_interleave(torch.arange(A.nb, device=A.cols.device),
+                                   torch.diff(A.rowptr.long()))
+    blocks = A.vals64.reshape(A.nnzb, b, b)
+    diag = blocks[A.cols.long() == rows]
+    tr = float(torch.diagonal(diag, dim1=1, dim2=2).sum().item())
+    fro2 = float((A.vals64 * A.vals64).sum().item())
+    mu = tr / N
+    sig = math.sqrt(max(fro2 / N - mu * mu, 0.0))
+    half = 2.0 * sig * math.sqrt(p / N)
+    lo, hi = max(mu - half, 0.0), mu + half
+    return torch.tensor([lo, hi], dtype=torch.float64, device=A.cols.device)
+
+
 def _deflate(Y, Xl):
     """Y <- Y - Xl (Xl^T Y), twice (classical Gram-Schmidt with reorthogonalisation)."""
     if Xl is None or Xl.shape[1] == 0:
@@ -281,13 +303,16 @@ def chfsi(A: BSR, k, precision="fp64", tol=1e-9, max_iter=300, guard=None, seed=
 
     t0 = time.perf_counter()
     X = X0.contiguous() if X0 is not None else fresh(p, torch.float64)
+    # a random block opens with a filter pass bounded by where its Ritz
+    # values would lie, instead of a Rayleigh-Ritz step on the unfiltered block
+    opening = None if X0 is not None else (_random_block_bracket(A, p), 1.0)
     if precision == "mixed":
         # FP32 filter body with locking; hands over at 1e-5, a few times above
         # the FP32 residual floor (a phase that reaches the floor first
         # stalls there)
         X, theta, it, _ = _run_phase(A, X.to(torch.float32).contiguous(), k, ub, torch.float32,
                                      1e-5, max_iter, 300.0, True, fresh, stats, verbose, "fp32",
-                                     stall_exit=True, scale=scale)
+                                     stall_exit=True, scale=scale, ritz=opening)
         stats["iters_fp32"] = it
         stats["t_fp32"] = time.perf_counter() - t0
         # FP64 polish with locking: the FP32 block is already accurate to
@@ -303,7 +328,7 @@ def chfsi(A: BSR, k, precision="fp64", tol=1e-9, max_iter=300, guard=None, seed=
     elif precision == "fp64":
         X, theta, it, done = _run_phase(A, X.to(torch.float64).contiguous(), k, ub, torch.float64,
                                         tol, max_iter, 1e4, True, fresh, stats, verbose, "fp64",
-                                        scale=scale)
+                                        scale=scale, ritz=opening)
         if not done:
             X, theta, _, _ = _run_phase(A, X, k, ub, torch.float64, tol, 3, 1e4, False, fresh,
                                         stats, verbose, "fp64-final", m_max=16, scale=scale)

@@ -196,8 +196,9 @@ int rvgp_gemm(int mode, int transA, int transB, int M, int N, int K, double alph
 /* Symmetric product C = A^T B (M x M, A and B K x M row-major, f64 out) for
  * Grams known to be symmetric (A^T A, or A^T (S A) with S symmetric): only
  * the upper 128 x 128 tiles are computed, the lower triangle is mirrored from
- * the upper.  mode F64 or F32_ACC64; split_k / ws as rvgp_gemm (workspace
- * from rvgp_gemm_ws_bytes(mode, M, M, K, split_k)).  The Rayleigh-Ritz Grams
+ * the upper.  mode F64, F32_ACC64, or F32: f32 in, FP32 MFMA over K-chunks of
+ * at most 4096 rows, chunk partials summed in f64 (f64 out); split_k / ws as
+ * rvgp_gemm (workspace from rvgp_gemm_ws_bytes(mode, M, M, K, split_k)).  The Rayleigh-Ritz Grams
  * of the eigensolver that replaces eigsh (geometry.py:73). */
 int rvgp_gemm_sym(int mode, int M, int K, const void* A, int lda, const void* B, int ldb,
                   double* C, int ldc, int split_k, void* ws, size_t ws_bytes, void* stream);

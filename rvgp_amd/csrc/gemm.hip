@@ -628,6 +628,71 @@ static int gemm_launch(int transA, int transB, int M, int N, int K, double alpha
     return RVGP_OK;
 }
 
+// Symmetric Gram of FP32 blocks on the FP32 matrix cores (twice the FP64
+// MFMA rate): K is cut into chunks of at most kF32Chunk rows, each chunk
+// accumulated in FP32 by the 128 x 128 kernel (upper tiles only), and the
+// chunk partials are summed in FP64 in chunk order.  A chunk of 4096 rows
+// carries ~sqrt(4096) * 6e-8 = 4e-6 relative rounding, and the FP64 sum of
+// the independent chunk errors ~1e-6 -- below the FP32 data's own level in
+// a Rayleigh-Ritz step of an FP32 block.
+constexpr int kF32Chunk = 4096;
+
+static GemmPlan gram_f32_plan(int M, int K, int split_k) {
+    GemmPlan p = gemm_plan(RVGP_GEMM_F32, M, M, K, Tr<Mode::F32>::BK, split_k);
+    const int need = (K + kF32Chunk - 1) / kF32Chunk;
+    if (p.small || p.split < need) {
+        p.small = false;
+        p.big = M >= 96;
+        p.split = need < 1 ? 1 : need;
+        int kchunk = (K + p.split - 1) / p.split;
+        kchunk = ((kchunk + Tr<Mode::F32>::BK - 1) / Tr<Mode::F32>::BK) * Tr<Mode::F32>::BK;
+        p.kchunk = kchunk < Tr<Mode::F32>::BK ? Tr<Mode::F32>::BK : kchunk;
+        p.split = K > 0 ? (K + p.kchunk - 1) / p.kchunk : 1;
+    }
+    return p;
+}
+
+__global__ void splitk_reduce_f32_f64(int M, int N, int S, const float* __restrict__ part,
+                                      double* __restrict__ C, int ldc) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)M * N) return;
+    double acc = 0.0;
+    for (int z = 0; z < S; ++z) acc += (double)part[(size_t)z * M * N + e];
+    const size_t gm = e / N, gn = e - gm * N;
+    C[gm * ldc + gn] = acc;
+}
+
+static int gram_f32_chunked(int M, int K, const float* A, int lda, const float* B, int ldb,
+                            double* C, int ldc, int split_k, void* ws, size_t ws_bytes,
+                            void* stream) {
+    RVGP_CHECK_ARG(M >= 0 && K >= 0 && ldc >= M && lda >= M && ldb >= M,
+                   "rvgp_gemm_sym: bad sizes");
+    if (M == 0) return RVGP_OK;
+    hipStream_t s = as_stream(stream);
+    const GemmPlan pl = gram_f32_plan(M, K, split_k);
+    const size_t need = sizeof(float) * (size_t)pl.split * M * M;
+    RVGP_CHECK_ARG(ws != nullptr && ws_bytes >= need,
+                   "rvgp_gemm_sym: the FP32 Gram needs %zu bytes of workspace", need);
+    float* part = static_cast<float*>(ws);
+    if (pl.big) {
+        dim3 grid(cdiv(M, BBM), cdiv(M, BBN), pl.split);
+        hipLaunchKernelGGL(gemm_big_kernel<Mode::F32>, grid, dim3(GT), 0, s, 1, 0, M, M, K,
+                           pl.kchunk, 1.0f, A, lda, B, ldb, nullptr, 0.0f, nullptr, M, part, 1);
+    } else {
+        dim3 grid(cdiv(M, BM), cdiv(M, BN), pl.split);
+        hipLaunchKernelGGL(gemm_kernel<Mode::F32>, grid, dim3(GT), 0, s, 1, 0, M, M, K, pl.kchunk,
+                           1.0f, A, lda, B, ldb, nullptr, 0.0f, nullptr, M, part, 1);
+    }
+    RVGP_LAUNCH_CHECK();
+    hipLaunchKernelGGL(splitk_reduce_f32_f64, dim3(cdiv((size_t)M * M, 256)), dim3(256), 0, s, M, M,
+                       pl.split, part, C, ldc);
+    RVGP_LAUNCH_CHECK();
+    hipLaunchKernelGGL(mirror_lower<double>, dim3(cdiv((size_t)M * M, 256)), dim3(256), 0, s, M, C,
+                       ldc);
+    RVGP_LAUNCH_CHECK();
+    return RVGP_OK;
+}
+
 }  // namespace rvgp
 
 using namespace rvgp;
@@ -660,14 +725,24 @@ extern "C" size_t rvgp_gemm_ws_bytes(int mode, int M, int N, int K, int split_k)
     const int BK = mode == RVGP_GEMM_BF16 ? Tr<Mode::BF16>::BK : Tr<Mode::F64>::BK;
     const size_t acc = (mode == RVGP_GEMM_F64 || mode == RVGP_GEMM_F32_ACC64) ? 8 : 4;
     const GemmPlan p = gemm_plan(mode, M, N, K, BK, split_k);
-    return p.split > 1 ? acc * (size_t)p.split * M * N : 0;
+    size_t bytes = p.split > 1 ? acc * (size_t)p.split * M * N : 0;
+    if (mode == RVGP_GEMM_F32 && M == N) {  // also covers rvgp_gemm_sym's chunked FP32 Gram
+        const GemmPlan q = gram_f32_plan(M, K, split_k);
+        const size_t b2 = sizeof(float) * (size_t)q.split * M * N;
+        if (b2 > bytes) bytes = b2;
+    }
+    return bytes;
 }
 
 extern "C" int rvgp_gemm_sym(int mode, int M, int K, const void* A, int lda, const void* B,
                              int ldb, double* C, int ldc, int split_k, void* ws, size_t ws_bytes,
                              void* stream) {
-    RVGP_CHECK_ARG(mode == RVGP_GEMM_F64 || mode == RVGP_GEMM_F32_ACC64,
-                   "rvgp_gemm_sym: mode must be F64 or F32_ACC64");
+    RVGP_CHECK_ARG(mode == RVGP_GEMM_F64 || mode == RVGP_GEMM_F32_ACC64 || mode == RVGP_GEMM_F32,
+                   "rvgp_gemm_sym: mode must be F64, F32_ACC64 or F32");
+    if (mode == RVGP_GEMM_F32)
+        return gram_f32_chunked(M, K, static_cast<const float*>(A), lda,
+                                static_cast<const float*>(B), ldb, C, ldc, split_k, ws, ws_bytes,
+                                stream);
     if (mode == RVGP_GEMM_F64)
         return gemm_launch<Mode::F64>(1, 0, M, M, K, 1.0, A, lda, B, ldb, nullptr, 0.0, C, ldc,
                                       split_k, ws, ws_bytes, stream, 0, 1);

@@ -61,12 +61,17 @@ def gemm(A, B, C=None, transA=False, transB=False, alpha=1.0, beta=0.0, kscale=N
 MODE_NAMES = {GEMM_F64: "f64", GEMM_F32: "f32", GEMM_F32_ACC64: "f32acc64", GEMM_BF16: "bf16"}
 
 
-def gemm_sym(A, B):
+def gemm_sym(A, B, fp32_mfma=True):
     """C = A^T B (f64) for a product known to be symmetric: upper tiles only,
-    lower triangle mirrored (rvgp_gemm_sym).  A, B: (K, M), f64 or f32."""
+    lower triangle mirrored (rvgp_gemm_sym).  A, B: (K, M), f64 or f32.  f32
+    blocks run on the FP32 matrix cores in K-chunks of <= 4096 rows summed in
+    FP64 (fp32_mfma=True), or widened to FP64 on the FP64 matrix cores."""
     K, M = A.shape
     assert B.shape == (K, M) and A.dtype == B.dtype
-    mode = GEMM_F64 if A.dtype == torch.float64 else GEMM_F32_ACC64
+    if A.dtype == torch.float64:
+        mode = GEMM_F64
+    else:
+        mode = GEMM_F32 if fp32_mfma else GEMM_F32_ACC64
     C = torch.empty((M, M), dtype=torch.float64, device=A.device)
     ws_bytes = _lib.lib().rvgp_gemm_ws_bytes(mode, M, M, K, 0)
     ws = torch.empty(ws_bytes, dtype=torch.uint8, device=A.device) if ws_bytes > 0 else None

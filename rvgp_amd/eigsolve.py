@@ -37,6 +37,17 @@ from . import dense
 from .sparse import BSR, spmm
 
 
+# block sizes whose FP32 phase filters twice (Cholesky QR in between) while the
+# degree is capped by conditioning
+DOUBLE_FILTER_BLOCKS = (1,)
+# largest amplification spread a filter may put across the block: FP32
+# blocks keep ~300 (the weakest directions then carry ~2e-5 relative
+# precision); FP64 blocks FP64_MAX_RATIO (the Rayleigh-Ritz Gram's condition
+# is its square, which the Cholesky basis must still resolve)
+FP32_MAX_RATIO = 300.0
+FP64_MAX_RATIO = 1e4
+
+
 class EigStats(dict):
     pass
 
@@ -252,7 +263,8 @@ def _run_phase(A, X, k, ub, wdt, ptol, max_iter, max_ratio, lock, fresh, stats, 
         tf = time.perf_counter()
         Y = _cheb_filter(A, Xa, m, cut, ub, low)
         stats["spmm"] += m
-        if (A.b == 1 and wdt == torch.float32 and Xl.shape[1] == 0 and m_need is not None
+        if (A.b in DOUBLE_FILTER_BLOCKS and wdt == torch.float32 and Xl.shape[1] == 0
+                and m_need is not None
                 and m_need > m):
             # FP32 phase of a scalar operator, degree capped by the block's
             # conditioning rather than by the residual target:
@@ -311,7 +323,7 @@ def chfsi(A: BSR, k, precision="fp64", tol=1e-9, max_iter=300, guard=None, seed=
         # the FP32 residual floor (a phase that reaches the floor first
         # stalls there)
         X, theta, it, _ = _run_phase(A, X.to(torch.float32).contiguous(), k, ub, torch.float32,
-                                     1e-5, max_iter, 300.0, True, fresh, stats, verbose, "fp32",
+                                     1e-5, max_iter, FP32_MAX_RATIO, True, fresh, stats, verbose, "fp32",
                                      stall_exit=True, scale=scale, ritz=opening)
         stats["iters_fp32"] = it
         stats["t_fp32"] = time.perf_counter() - t0
@@ -322,15 +334,15 @@ def chfsi(A: BSR, k, precision="fp64", tol=1e-9, max_iter=300, guard=None, seed=
                                         ritz=(theta, stats["last_worst"]), scale=scale)
         if not done:
             # an unlocked Rayleigh-Ritz over the whole block orders and refines
-            X, theta, _, _ = _run_phase(A, X, k, ub, torch.float64, tol, 3, 1e4, False, fresh,
+            X, theta, _, _ = _run_phase(A, X, k, ub, torch.float64, tol, 3, FP64_MAX_RATIO, False, fresh,
                                         stats, verbose, "fp64-final", m_max=16, scale=scale)
         stats["iters_polish"] = it
     elif precision == "fp64":
         X, theta, it, done = _run_phase(A, X.to(torch.float64).contiguous(), k, ub, torch.float64,
-                                        tol, max_iter, 1e4, True, fresh, stats, verbose, "fp64",
+                                        tol, max_iter, FP64_MAX_RATIO, True, fresh, stats, verbose, "fp64",
                                         scale=scale, ritz=opening)
         if not done:
-            X, theta, _, _ = _run_phase(A, X, k, ub, torch.float64, tol, 3, 1e4, False, fresh,
+            X, theta, _, _ = _run_phase(A, X, k, ub, torch.float64, tol, 3, FP64_MAX_RATIO, False, fresh,
                                         stats, verbose, "fp64-final", m_max=16, scale=scale)
     else:
         raise ValueError("precision must be 'fp64' or 'mixed'")

@@ -112,11 +112,15 @@ def test_gemm_sym_matches_full_product(K, M):
     W = Y + 0.1 * rng.normal(size=(K, M))
     for A, B in ((Y, Y), (Y, W)):
         ref = A.T @ B
-        for dt, tol in ((torch.float64, 1e-11), (torch.float32, 1e-10)):
+        # FP32 blocks: widened to FP64 on the FP64 matrix cores (exact
+        # products, FP64 sums), or FP32 MFMA in <= 4096-row chunks summed in
+        # FP64 (~sqrt(4096) * 6e-8 per chunk)
+        for dt, f32mfma, tol in ((torch.float64, False, 1e-11), (torch.float32, False, 1e-10),
+                                 (torch.float32, True, 1e-5)):
             if dt == torch.float32:
                 A32, B32 = A.astype(np.float32).astype(np.float64), B.astype(np.float32).astype(np.float64)
                 ref = A32.T @ B32
-            C = dense.gemm_sym(_t(A, dt), _t(B, dt)).cpu().numpy()
+            C = dense.gemm_sym(_t(A, dt), _t(B, dt), fp32_mfma=f32mfma).cpu().numpy()
             iu = np.triu_indices(M)
             np.testing.assert_allclose(C[iu], ref[iu], rtol=0, atol=tol * np.abs(ref).max())
             assert np.array_equal(C, C.T)

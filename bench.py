@@ -269,7 +269,8 @@ def bench_batched(args):
     e1.synchronize()
     step_ms = e0.elapsed_time(e1) / 50
     e0.record()
-    gp2 = BatchedGPR(d, Vd, sets, theta0=th0, gram_tile=gp.gram_tile, panel=gp.panel)
+    gp2 = BatchedGPR(d, Vd, sets, theta0=th0, gram_tile=gp.gram_tile,
+                     panel=(gp.panel, gp.threads))
     e1.record()
     e1.synchronize()
     setup_ms = e0.elapsed_time(e1)
@@ -307,7 +308,7 @@ def bench_batched(args):
         "step_ms": round(step_ms, 4), "setup_ms": round(setup_ms, 3),
         "step_fp64_tflops": round(step_flops / (step_ms * 1e9), 3),
         "gram_rows": rows, "gram_fp64_tflops_incl_rhs": round(gram_flops / (setup_ms * 1e9), 3),
-        "autotune": {"gram_tile": gp.gram_tile, "panel": gp.panel},
+        "autotune": {"gram_tile": gp.gram_tile, "panel": gp.panel, "threads": gp.threads},
         "sequential_single_fit_s": round(t_seq_all, 3),
         "speedup_vs_sequential": round(t_seq_all / t_fit, 2),
         "smoothing_s": round(t_smooth, 3),

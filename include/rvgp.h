@@ -312,7 +312,8 @@ int rvgp_batch_rhs(int F, int k, int d, const double* U, int ldu, const int* nod
  * rvgp_ws_finish into out (F, 9), and -- when do_adam -- one Adam update of
  * u_f with state m1, m2 (F, 4) and step counters t (F).  M (F): observations
  * per fit (ambient rows).  panel = 16 or 32 (Cholesky block width; needs
- * 8*(panel+1)*(k+panel) bytes of LDS <= 160 KiB).  ws: rvgp_batch_step_ws_bytes
+ * 8*(panel+1)*(k+panel) bytes of LDS <= 159 KiB), threads = 512 or 1024 per
+ * fit workgroup (the autotuner's choice).  ws: rvgp_batch_step_ws_bytes
  * bytes; per fit it holds [L | L^-1] (k x 2k) then S, dS, gS, diag(A^-1),
  * L^-1 b, w = A^-1 b, G w.  status (F): RVGP_ST_NOT_SPD for a fit whose A
  * is not positive definite. */
@@ -320,8 +321,9 @@ size_t rvgp_batch_step_ws_bytes(int F, int k);
 int rvgp_batch_step(int F, int k, int typ, const double* evals, double n_rows, const double* G,
                     const double* b, const double* yy, const double* M, const double* lower,
                     double* u, const int* trainable, double* m1, double* m2, double* t,
-                    int do_adam, double lr, double b1, double b2, double eps, int panel, int grid,
-                    double* ws, size_t ws_bytes, double* out, int* status, void* stream);
+                    int do_adam, double lr, double b1, double b2, double eps, int panel,
+                    int threads, int grid, double* ws, size_t ws_bytes, double* out, int* status,
+                    void* stream);
 
 /* ---------------------------------------------------------------------
  * kernel='rbf' branch of RVGP.fit (RVGP/main.py:33-37): gpflow.kernels.RBF

@@ -63,8 +63,8 @@ SIGNATURES = {
     "rvgp_rbf_kernel": (I, [I, I, P, I, P, P, P, I, I, P, I, P]),
     "rvgp_rbf_grad_ws_bytes": (S, [I]),
     "rvgp_rbf_grad": (I, [I, I, P, I, P, I, P, I, P, P, P, P, P, P, S, P, P]),
-    "rvgp_batch_step": (I, [I, I, I, P, D, P, P, P, P, P, P, P, P, P, P, I, D, D, D, D, I, I, P, S,
-                            P, P, P]),
+    "rvgp_batch_step": (I, [I, I, I, P, D, P, P, P, P, P, P, P, P, P, P, I, D, D, D, D, I, I, I,
+                            P, S, P, P, P]),
 }
 
 RVGP_EINVAL, RVGP_ERUNTIME, RVGP_ENOMEM, RVGP_EHIP = -1, -2, -3, -4

@@ -117,17 +117,21 @@ class BatchedGPR:
             grid = torch.cuda.get_device_properties(dev).multi_processor_count
         self.grid = int(grid)
         if panel is None:
-            panel = autotune.choose("batch_step", f"F{F},k{k},grid{self.grid}", (16, 32),
-                                    lambda p: self._step(0, 0.0, panel=p))
-        self.panel = int(panel)
+            panel = autotune.choose("batch_step", f"F{F},k{k},grid{self.grid}",
+                                    ((16, 512), (16, 1024), (32, 512), (32, 1024)),
+                                    lambda c: self._step(0, 0.0, panel=c))
+        # panel: the Cholesky block width, or (width, threads per workgroup)
+        self.panel, self.threads = (int(panel), 512) if np.isscalar(panel) else map(int, panel)
 
     # ------------------------------------------------------------------ steps
     def _step(self, do_adam, lr, b1=0.9, b2=0.999, eps=1e-8, panel=None):
+        nb, nt = (self.panel, self.threads) if panel is None else (
+            (int(panel), 512) if np.isscalar(panel) else panel)
         _lib.call("rvgp_batch_step", self.F, self.k, self.typ, _lib.ptr(self.evals), self.n_rows,
                   _lib.ptr(self.G), _lib.ptr(self.b), _lib.ptr(self.yy), _lib.ptr(self.M),
                   _lib.ptr(self.lower), _lib.ptr(self.u), _lib.ptr(self.trainable),
                   _lib.ptr(self.m1), _lib.ptr(self.m2), _lib.ptr(self.t), int(do_adam), float(lr),
-                  float(b1), float(b2), float(eps), int(panel or self.panel), self.grid,
+                  float(b1), float(b2), float(eps), int(nb), int(nt), self.grid,
                   _lib.ptr(self.ws), self.ws_bytes, _lib.ptr(self.out), _lib.ptr(self.status),
                   _lib.stream())
 

@@ -168,8 +168,10 @@ __global__ __launch_bounds__(256) void batch_rhs_kernel(int F, int k, int d,
 }
 
 // ------------------------------------------------------------------ step
-constexpr int ST = 512;  // threads per fit workgroup (8 waves)
-
+// Threads per fit workgroup (ST = 512 or 1024, a template parameter chosen
+// by the autotuner): more waves hide more of each panel step's memory
+// latency; fewer leave more registers per thread.
+template <int ST>
 __device__ double wg_sum(double v, double* sh) {
     v = wave_sum(v);
     const int w = threadIdx.x >> 6;
@@ -186,7 +188,7 @@ __device__ double wg_sum(double v, double* sh) {
 // S, dS (3k), gS, ainv, t, w, Gw (9k doubles).
 __host__ __device__ inline size_t step_ws_doubles(int k) { return 2 * (size_t)k * k + 9 * (size_t)k; }
 
-template <int NB>
+template <int NB, int ST>
 __global__ __launch_bounds__(ST) void batch_step_kernel(
     int F, int k, int typ, const double* __restrict__ lam, double n_rows,
     const double* __restrict__ G, const double* __restrict__ bvec, const double* __restrict__ yy,
@@ -244,9 +246,9 @@ __global__ __launch_bounds__(ST) void batch_step_kernel(
             snu += r * dn;
             ska += r * dk;
         }
-        sr = wg_sum(sr, sh);
-        snu = wg_sum(snu, sh) / sr;
-        ska = wg_sum(ska, sh) / sr;
+        sr = wg_sum<ST>(sr, sh);
+        snu = wg_sum<ST>(snu, sh) / sr;
+        ska = wg_sum<ST>(ska, sh) / sr;
         for (int j = tid; j < k; j += ST) {
             double r, dn, dk;
             if (typ == 0) {
@@ -510,11 +512,11 @@ __global__ __launch_bounds__(ST) void batch_step_kernel(
             tr += ainv[j] / sj;
             gS[j] = 0.5 * (1.0 / sj - s2 * ainv[j] / (sj * sj) - wj * wj / (sj * sj));
         }
-        bw = wg_sum(bw, sh);
-        wGw = wg_sum(wGw, sh);
-        ld = wg_sum(ld, sh);
-        lS = wg_sum(lS, sh);
-        tr = wg_sum(tr, sh);
+        bw = wg_sum<ST>(bw, sh);
+        wGw = wg_sum<ST>(wGw, sh);
+        ld = wg_sum<ST>(ld, sh);
+        lS = wg_sum<ST>(lS, sh);
+        tr = wg_sum<ST>(tr, sh);
         const double y2 = yy[f];
         const double nlml = 0.5 * (y2 - bw) / s2 + 0.5 * ((Mf - k) * log(s2) + lS + 2.0 * ld) +
                             0.5 * Mf * log(6.283185307179586);
@@ -527,9 +529,9 @@ __global__ __launch_bounds__(ST) void batch_step_kernel(
             g1 += gS[j] * dS[k + j];
             g2 += gS[j] * dS[2 * k + j];
         }
-        g0 = wg_sum(g0, sh);
-        g1 = wg_sum(g1, sh);
-        g2 = wg_sum(g2, sh);
+        g0 = wg_sum<ST>(g0, sh);
+        g1 = wg_sum<ST>(g1, sh);
+        g2 = wg_sum<ST>(g2, sh);
         if (tid < 4) {
             const double gth = tid == 0 ? g0 : tid == 1 ? g1 : tid == 2 ? g2 : gs2;
             const double x = uf[tid];
@@ -559,7 +561,7 @@ __global__ __launch_bounds__(ST) void batch_step_kernel(
     }
 }
 
-template <int NB>
+template <int NB, int ST>
 static size_t step_lds_bytes(int k) {
     // Ld, then the panel P (k x (NB + 1)), whose space also holds the
     // per-wave partial sums (ST / 64) x k of the final row passes
@@ -568,13 +570,13 @@ static size_t step_lds_bytes(int k) {
     return sizeof(double) * ((size_t)NB * (NB + 1) + pn);
 }
 
-template <int NB>
+template <int NB, int ST>
 static int launch_step(int grid, int F, int k, int typ, const double* lam, double n_rows,
                        const double* G, const double* b, const double* yy, const double* M,
                        const double* lower, double* u, const int* trainable, double* m1,
                        double* m2, double* t, int do_adam, double lr, double b1, double b2,
                        double eps, double* ws, double* out, int* status, hipStream_t s) {
-    const size_t lds = step_lds_bytes<NB>(k);
+    const size_t lds = step_lds_bytes<NB, ST>(k);
     // 160 KiB of LDS per workgroup, less 1 KiB kept for the static arrays
     constexpr size_t kMaxDyn = 160 * 1024 - 1024;
     RVGP_CHECK_ARG(lds <= kMaxDyn, "rvgp_batch_step: k = %d needs %zu bytes of LDS with panel "
@@ -584,11 +586,11 @@ static int launch_step(int grid, int F, int k, int typ, const double* lam, doubl
     // (not per launch: the launch may be under HIP graph capture)
     static bool opted_in = false;
     if (!opted_in) {
-        RVGP_CHECK_HIP(hipFuncSetAttribute((const void*)batch_step_kernel<NB>,
+        RVGP_CHECK_HIP(hipFuncSetAttribute((const void*)batch_step_kernel<NB, ST>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxDyn));
         opted_in = true;
     }
-    hipLaunchKernelGGL(batch_step_kernel<NB>, dim3(grid), dim3(ST), lds, s, F, k, typ, lam, n_rows,
+    hipLaunchKernelGGL((batch_step_kernel<NB, ST>), dim3(grid), dim3(ST), lds, s, F, k, typ, lam, n_rows,
                        G, b, yy, M, lower, u, trainable, m1, m2, t, do_adam, lr, b1, b2, eps, ws,
                        out, status);
     RVGP_LAUNCH_CHECK();
@@ -652,20 +654,23 @@ extern "C" int rvgp_batch_step(int F, int k, int typ, const double* evals, doubl
                                const double* M, const double* lower, double* u,
                                const int* trainable, double* m1, double* m2, double* t,
                                int do_adam, double lr, double b1, double b2, double eps,
-                               int panel, int grid, double* ws, size_t ws_bytes, double* out,
-                               int* status, void* stream) {
+                               int panel, int threads, int grid, double* ws, size_t ws_bytes,
+                               double* out, int* status, void* stream) {
     RVGP_CHECK_ARG(F >= 1 && k >= 1, "rvgp_batch_step: bad sizes");
     RVGP_CHECK_ARG(typ == 0 || typ == 1, "rvgp_batch_step: typ must be 0 (matern) or 1 (se)");
     RVGP_CHECK_ARG(ws_bytes >= rvgp_batch_step_ws_bytes(F, k), "rvgp_batch_step: workspace too small");
     RVGP_CHECK_ARG(grid >= 1, "rvgp_batch_step: grid >= 1");
     const int g = grid < F ? grid : F;
     hipStream_t s = as_stream(stream);
-    if (panel == 16)
-        return launch_step<16>(g, F, k, typ, evals, n_rows, G, b, yy, M, lower, u, trainable, m1,
-                               m2, t, do_adam, lr, b1, b2, eps, ws, out, status, s);
-    if (panel == 32)
-        return launch_step<32>(g, F, k, typ, evals, n_rows, G, b, yy, M, lower, u, trainable, m1,
-                               m2, t, do_adam, lr, b1, b2, eps, ws, out, status, s);
-    RVGP_CHECK_ARG(false, "rvgp_batch_step: panel must be 16 or 32 (got %d)", panel);
+#define RVGP_STEP(NB_, ST_)                                                                        \
+    return launch_step<NB_, ST_>(g, F, k, typ, evals, n_rows, G, b, yy, M, lower, u, trainable, m1, \
+                                 m2, t, do_adam, lr, b1, b2, eps, ws, out, status, s)
+    if (panel == 16 && threads == 512) RVGP_STEP(16, 512);
+    if (panel == 16 && threads == 1024) RVGP_STEP(16, 1024);
+    if (panel == 32 && threads == 512) RVGP_STEP(32, 512);
+    if (panel == 32 && threads == 1024) RVGP_STEP(32, 1024);
+#undef RVGP_STEP
+    RVGP_CHECK_ARG(false, "rvgp_batch_step: (panel, threads) must be 16 or 32 and 512 or 1024 "
+                          "(got %d, %d)", panel, threads);
     return RVGP_EINVAL;
 }

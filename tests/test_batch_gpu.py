@@ -88,7 +88,7 @@ def test_batch_statistics_equal_ambient_products(sphere, tile, prec):
         assert abs(gp.yy[f].item() - float((y * y).sum())) < 1e-12 * float((y * y).sum())
 
 
-@pytest.mark.parametrize("panel", [16, 32])
+@pytest.mark.parametrize("panel", [16, 32, (16, 1024), (32, 1024)])
 @pytest.mark.parametrize("which", ["sphere", "torus3"])
 def test_batch_nlml_and_adjoint_vs_oracle(request, panel, which):
     """One launch evaluates every fit: NLML and gradient w.r.t. (nu, kappa,
@@ -174,3 +174,17 @@ def test_fit_batched_and_transform_vs_oracle(torus3):
     # the NLML of every fit went down from the initial hyperparameters
     gp0 = RVGP.fit_batched(d, sets, vectors=Vd, epochs=0)
     assert np.all(gp.training_loss() < gp0.training_loss())
+
+
+def test_batch_persistent_grid_smaller_than_fit_count(sphere):
+    """With fewer workgroups than fits each workgroup loops over fits: the
+    result equals the one-workgroup-per-fit launch."""
+    from rvgp_amd.batch import BatchedGPR
+    d = sphere
+    Vd, sets, th = fields_and_sets(d, 5, [3, 30, 90, 200, 410], seed=13)
+    a = BatchedGPR(d, Vd, sets, theta0=th, gram_tile=64, panel=32)
+    b = BatchedGPR(d, Vd, sets, theta0=th, gram_tile=64, panel=32, grid=2)
+    a.fit(epochs=4, use_graph=False)
+    b.fit(epochs=4, use_graph=False)
+    np.testing.assert_array_equal(a.out.cpu().numpy(), b.out.cpu().numpy())
+    np.testing.assert_array_equal(a.u.cpu().numpy(), b.u.cpu().numpy())

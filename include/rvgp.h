@@ -63,7 +63,9 @@ int rvgp_fps_diam2(const double* x, int n, int D, double* diam2, void* stream);
  * workgroup.  n_iter = N, or n when N is None.  If use_spacing != 0 the loop
  * stops at the first i with lambdas[i]/sqrt(diam2) < spacing and writes i to
  * n_sel[0]; otherwise n_sel[0] = n_iter.  ds: workspace of n doubles.
- * perm: (n_iter,) int32, lambdas: (n_iter,) f64. */
+ * perm: (n_iter,) int32, lambdas: (n_iter,) f64.  1 <= D <= 8192 (the
+ * spacing mode's rvgp_fps_diam2 takes D <= 64); main.py:59-60 runs it on
+ * the spectral rows (D = n_eigenpairs) to pick SGPR inducing points. */
 int rvgp_fps(const double* x, int n, int D, int n_iter, int start_idx, double spacing,
              int use_spacing, const double* diam2, double* ds, int* perm, double* lambdas,
              int* n_sel, void* stream);
@@ -349,6 +351,21 @@ int rvgp_rbf_grad(int M, int nrhs, const double* Kinv, int ldk, const double* AA
                   const double* G, int ldg, const double* sqn, const double* theta, const double* ya,
                   const double* logdet_half, const double* u, double* ws, size_t ws_bytes,
                   double* out, void* stream);
+
+/* ---------------------------------------------------------------------
+ * n_inducing_points branch of RVGP.fit (RVGP/main.py:54-67): GPflow SGPR
+ * with the manifold kernel.  -ELBO and its adjoint from the weight-space
+ * statistics G (k x k), b (k), yy and GPflow's factors of the inducing rows
+ * Z (m x k): Kuu = Z S Z^T + jitter I = L L^T, Q = L^-1 Z, R1 = Q S G,
+ * B = I + R1 S Q^T / s2 = L_B L_B^T, Qb = L_B^-1 Q, R1b = L_B^-1 R1,
+ * W1 = R1 S Q^T Q, cc = L_B^-1 Q S b, zw = Qb^T cc, hw = R1b^T cc; scal =
+ * [sum log diag L_B, unused, |cc|^2]; S, dS of rvgp_spectral_filter; out (9)
+ * as rvgp_ws_finish.  N = training rows. */
+int rvgp_sgpr_finish(int m, int k, double N, const double* Q, const double* Qb, const double* R1,
+                     const double* R1b, const double* W1, const double* G, const double* b,
+                     const double* zw, const double* hw, const double* S, const double* dS,
+                     const double* scal, const double* yy, double s2, const double* u, double* gS,
+                     double* out, void* stream);
 
 #ifdef __cplusplus
 }

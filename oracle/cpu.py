@@ -443,6 +443,60 @@ def gpr_predict(Phi, y, S, noise, Phi_star):
 
 
 # --------------------------------------------------------------------------
+# n_inducing_points branch (main.py:54-67): GPflow's SGPR, the collapsed
+# (Titsias) bound, with the spectral kernel Phi diag(S) Phi^T
+# --------------------------------------------------------------------------
+SGPR_JITTER = 1e-6  # gpflow.config.default_jitter()
+
+
+def sgpr_nlml(Phi, y, S, noise, Z, jitter=SGPR_JITTER):
+    """-ELBO of gpflow.models.SGPR, written the way SGPR.elbo forms it:
+    A = L^-1 K_uf / sigma, B = I + A A^T, c = L_B^-1 A y / sigma."""
+    N = Phi.shape[0]
+    y = y.reshape(N, -1)
+    R = y.shape[1]
+    m = Z.shape[0]
+    Kdiag = np.einsum("ij,j,ij->i", Phi, S, Phi)
+    kuf = (Z * S) @ Phi.T
+    kuu = (Z * S) @ Z.T + jitter * np.eye(m)
+    L = np.linalg.cholesky(kuu)
+    sigma = np.sqrt(noise)
+    A = scipy.linalg.solve_triangular(L, kuf, lower=True) / sigma
+    AAT = A @ A.T
+    LB = np.linalg.cholesky(AAT + np.eye(m))
+    c = scipy.linalg.solve_triangular(LB, A @ y, lower=True) / sigma
+    bound = -0.5 * N * R * np.log(2 * np.pi)
+    bound -= R * np.log(np.diag(LB)).sum()
+    bound -= 0.5 * N * R * np.log(noise)
+    bound -= 0.5 * (y * y).sum() / noise
+    bound += 0.5 * (c * c).sum()
+    bound -= 0.5 * R * Kdiag.sum() / noise
+    bound += 0.5 * R * np.trace(AAT)
+    return -bound
+
+
+def sgpr_predict(Phi, y, S, noise, Z, Phi_star, jitter=SGPR_JITTER):
+    """gpflow.models.SGPR.predict_f(full_cov=False): mean (m*, R), variance (m*,)."""
+    N = Phi.shape[0]
+    y = y.reshape(N, -1)
+    m = Z.shape[0]
+    kuf = (Z * S) @ Phi.T
+    kuu = (Z * S) @ Z.T + jitter * np.eye(m)
+    kus = (Z * S) @ Phi_star.T
+    L = np.linalg.cholesky(kuu)
+    sigma = np.sqrt(noise)
+    A = scipy.linalg.solve_triangular(L, kuf, lower=True) / sigma
+    LB = np.linalg.cholesky(A @ A.T + np.eye(m))
+    c = scipy.linalg.solve_triangular(LB, A @ y, lower=True) / sigma
+    tmp1 = scipy.linalg.solve_triangular(L, kus, lower=True)
+    tmp2 = scipy.linalg.solve_triangular(LB, tmp1, lower=True)
+    mean = tmp2.T @ c
+    var = (np.einsum("ij,j,ij->i", Phi_star, S, Phi_star) + (tmp2 * tmp2).sum(0)
+           - (tmp1 * tmp1).sum(0))
+    return mean, var
+
+
+# --------------------------------------------------------------------------
 # kernel='rbf' branch (main.py:33-37): gpflow.kernels.RBF on evecs_L rows,
 # the D vector components as D output columns of one GPR
 # --------------------------------------------------------------------------

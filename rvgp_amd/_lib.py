@@ -60,6 +60,7 @@ SIGNATURES = {
     "rvgp_batch_rhs": (I, [I, I, I, P, I, P, P, P, P, P, P, P]),
     "rvgp_batch_step_ws_bytes": (S, [I, I]),
     "rvgp_rbf_params": (I, [P, P, P, P]),
+    "rvgp_sgpr_finish": (I, [I, I, D, P, P, P, P, P, P, P, P, P, P, P, P, P, D, P, P, P, P]),
     "rvgp_rbf_kernel": (I, [I, I, P, I, P, P, P, I, I, P, I, P]),
     "rvgp_rbf_grad_ws_bytes": (S, [I]),
     "rvgp_rbf_grad": (I, [I, I, P, I, P, I, P, I, P, P, P, P, P, P, S, P, P]),

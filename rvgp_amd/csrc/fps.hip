@@ -69,6 +69,7 @@ __global__ __launch_bounds__(kDiamRows) void diam2_kernel(const double* __restri
 
 // ---- the greedy loop ---------------------------------------------------
 constexpr int kFpsThreads = 1024;
+constexpr int kFpsMaxD = 8192;
 
 struct ArgMax {
     double v;
@@ -101,12 +102,12 @@ __global__ __launch_bounds__(kFpsThreads) void fps_kernel(
     const double* __restrict__ x, int n, int D, int n_iter, int start, double spacing,
     int use_spacing, const double* __restrict__ diam2, double* __restrict__ ds,
     int* __restrict__ perm, double* __restrict__ lambdas, int* __restrict__ n_sel) {
-    __shared__ double sx[64];
+    extern __shared__ double sx[];  // the selected point's D coordinates
     __shared__ double sv[kFpsThreads / kWave];
     __shared__ int si[kFpsThreads / kWave];
     const double diam = use_spacing ? sqrt(diam2[0]) : 1.0;
 
-    if (threadIdx.x < D) sx[threadIdx.x] = x[(size_t)start * D + threadIdx.x];
+    for (int c = threadIdx.x; c < D; c += kFpsThreads) sx[c] = x[(size_t)start * D + c];
     __syncthreads();
     double bv = -1.0;
     int bi = 0x7fffffff;
@@ -130,7 +131,7 @@ __global__ __launch_bounds__(kFpsThreads) void fps_kernel(
             count = it;
             break;
         }
-        if (threadIdx.x < D) sx[threadIdx.x] = x[(size_t)am.i * D + threadIdx.x];
+        for (int c = threadIdx.x; c < D; c += kFpsThreads) sx[c] = x[(size_t)am.i * D + c];
         __syncthreads();
         bv = -1.0;
         bi = 0x7fffffff;
@@ -168,11 +169,15 @@ extern "C" int rvgp_fps_diam2(const double* x, int n, int D, double* diam2, void
 extern "C" int rvgp_fps(const double* x, int n, int D, int n_iter, int start_idx, double spacing,
                         int use_spacing, const double* diam2, double* ds, int* perm,
                         double* lambdas, int* n_sel, void* stream) {
-    RVGP_CHECK_ARG(n >= 1 && D >= 1 && D <= 64, "rvgp_fps: need n >= 1 and 1 <= D <= 64");
+    // the selected point is staged in LDS: D <= 8192 (64 KiB); the spacing
+    // mode's diameter kernel takes D <= 64
+    RVGP_CHECK_ARG(n >= 1 && D >= 1 && D <= kFpsMaxD, "rvgp_fps: need n >= 1 and 1 <= D <= %d",
+                   kFpsMaxD);
     RVGP_CHECK_ARG(n_iter >= 1, "rvgp_fps: n_iter must be >= 1");
     RVGP_CHECK_ARG(start_idx >= 0 && start_idx < n, "rvgp_fps: start_idx out of range");
     RVGP_CHECK_ARG(!use_spacing || diam2 != nullptr, "rvgp_fps: spacing mode needs diam2");
-    hipLaunchKernelGGL(fps_kernel, dim3(1), dim3(kFpsThreads), 0, as_stream(stream), x, n, D,
+    hipLaunchKernelGGL(fps_kernel, dim3(1), dim3(kFpsThreads), sizeof(double) * D, as_stream(stream),
+                       x, n, D,
                        n_iter, start_idx, spacing, use_spacing, diam2, ds, perm, lambdas, n_sel);
     RVGP_LAUNCH_CHECK();
     return RVGP_OK;

@@ -117,10 +117,9 @@ def train_gp(data, train_ind=None, n_inducing_points=None, test_size=0.2, kernel
         nodes_tr, nodes_te = train_ind, train_ind[:0]
 
     set_default_positive_minimum(positivity_constraint)
-    if n_inducing_points is not None:
-        raise NotImplementedError("SGPR inducing points (main.py:59-67) are not on the GPU path")
-
     if isinstance(kernel, RBF):
+        if n_inducing_points is not None:
+            raise NotImplementedError("SGPR inducing points with kernel='rbf' are not built")
         return _train_rbf(data, kernel, nodes_tr, nodes_te, noise_variance, kernel_lengthscale,
                           kernel_variance, epochs, optimizer, reference_semantics, verbose, timings)
     if kernel_variance is not None:
@@ -145,6 +144,9 @@ def train_gp(data, train_ind=None, n_inducing_points=None, test_size=0.2, kernel
     noise0 = 1.0 if reference_semantics else float(noise_variance)
     lower = (kernel.nu.lower, kernel.kappa.lower, kernel.sigma_f.lower,
              GAUSSIAN_VARIANCE_LOWER_BOUND)
+    if n_inducing_points is not None:
+        return _train_sgpr(data, kernel, nodes_tr, nodes_te, Phi, y, V, int(n_inducing_points),
+                           lower, noise0, epochs, optimizer, reference_semantics, verbose, clock)
     eng = GPREngine(Phi, y, data.spectrum_device(), kernel.num_verticies, kernel.typ, form=form,
                     precision=precision, lower=lower)
     eng.set_theta([float(kernel.nu), float(kernel.kappa), float(kernel.sigma_f), noise0])
@@ -180,6 +182,52 @@ def train_gp(data, train_ind=None, n_inducing_points=None, test_size=0.2, kernel
         model.heldout_error = err
         if verbose:
             print("Relative l2 error is {}".format(err))
+    clock.tick("gp_heldout_predict")
+    return model
+
+
+class manifold_SGPR(manifold_GPR):
+    """The fitted n_inducing_points model (main.py:119-137): GPflow's SGPR
+    with the manifold kernel; ``inducing_nodes`` are the FPS-selected rows."""
+
+    def training_loss(self):
+        return self.engine.nlml_grad_u(self.engine.u)[0]
+
+
+def _train_sgpr(data, kernel, nodes_tr, nodes_te, Phi, y, V, n_inducing, lower, noise0, epochs,
+                optimizer, reference_semantics, verbose, clock):
+    """main.py:58-67: inducing rows = furthest_point_sampling(in_train,
+    N=n_inducing_points) of the training rows of evecs_Lc, then SGPR;
+    L-BFGS-B with maxiter=epochs (main.py:87-95)."""
+    from .geometry import furthest_point_sampling_device
+    from .sgpr import SGPREngine
+    if optimizer != "lbfgs":
+        raise NotImplementedError("SGPR is fitted with L-BFGS-B, as in the reference")
+    if y.dim() == 2 and y.shape[1] > 1:
+        raise ValueError("SGPR fits one field (vectors of shape (n, D))")
+    ind, _ = furthest_point_sampling_device(Phi, N=n_inducing)
+    Z = Phi[ind.long()].contiguous()
+    eng = SGPREngine(Phi, y, Z, data.spectrum_device(), kernel.num_verticies, kernel.typ,
+                     lower=lower)
+    eng.inducing_rows = ind.cpu().numpy()
+    eng.set_theta([float(kernel.nu), float(kernel.kappa), float(kernel.sigma_f), noise0])
+    eng.R = 1
+    clock.tick("gp_setup")
+    eng.fit_lbfgs(maxiter=epochs, disp=verbose)
+    clock.tick("gp_optimise")
+    th = eng.theta_host()
+    kernel.nu.assign(th[0])
+    kernel.kappa.assign(th[1])
+    kernel.sigma_f.assign(th[2])
+    model = manifold_SGPR(data, kernel, nodes_tr, eng, reference_semantics)
+    if len(nodes_te):
+        dev = eng.dev
+        mean, _ = model.predict_f(data.phi_rows(torch.as_tensor(nodes_te.astype(np.int32),
+                                                                device=dev)))
+        out_test = V[torch.as_tensor(nodes_te, device=dev).long()].reshape(-1, 1)
+        model.heldout_error = (out_test - mean).abs().mean().item()
+        if verbose:
+            print("Relative l2 error is {}".format(model.heldout_error))
     clock.tick("gp_heldout_predict")
     return model
 

@@ -284,3 +284,97 @@ def test_fit_rbf_kernel_vs_oracle_lbfgs():
     assert abs(gp.heldout_error - err) < 1e-8
     gp2 = RVGP.fit(d, train_ind=train, kernel='rbf', kernel_variance=0.5, epochs=50, verbose=False)
     assert gp2.hyperparameters["variance"] == pytest.approx(0.5, rel=1e-12)
+
+
+def test_fps_on_high_dimensional_rows_matches_oracle():
+    """FPS over spectral rows (D = n_eigenpairs, here 200 > 64) as main.py:59
+    runs it to pick SGPR inducing points: bit-exact indices and distances."""
+    from rvgp_amd import geometry
+    x = np.random.default_rng(41).normal(size=(3000, 200))
+    perm, lam = geometry.furthest_point_sampling(x, N=60, start_idx=7)
+    perm_ref, lam_ref = cpu.fps(x, N=60, start_idx=7)
+    np.testing.assert_array_equal(perm, perm_ref)
+    np.testing.assert_array_equal(lam, lam_ref)
+
+
+@pytest.mark.parametrize("m", [5, 20])
+def test_sgpr_engine_vs_oracle(m):
+    """SGPR bound and adjoint vs the oracle's GPflow-formula bound (value,
+    central differences of the oracle for the gradient) and predictions."""
+    import torch
+    from rvgp_amd.sgpr import SGPREngine
+    rng = np.random.default_rng(m)
+    Phi, y, lam = problem(M_nodes=100, k=25, seed=m)
+    Z = Phi[rng.choice(Phi.shape[0], m, replace=False)]
+    th = np.array([1.6, 3.0, 1.1, 0.2])
+    e = SGPREngine(torch.as_tensor(Phi, device="cuda"), torch.as_tensor(y, device="cuda"),
+                   torch.as_tensor(Z, device="cuda"), torch.as_tensor(lam, device="cuda"), 900.0)
+    e.set_theta(th)
+    out = e.evaluate().cpu().numpy()
+
+    def ref(t):
+        S, _ = cpu.spectral_filter(lam, t[0], t[1], t[2], 900.0)
+        return cpu.sgpr_nlml(Phi, y, S, t[3], Z)
+    r0 = ref(th)
+    assert abs(out[0] - r0) < 1e-9 * abs(r0)
+    h = 1e-5
+    for p in range(4):
+        tp, tm = th.copy(), th.copy()
+        tp[p] += h
+        tm[p] -= h
+        fd = (ref(tp) - ref(tm)) / (2 * h)
+        assert abs(fd - out[1 + p]) < 1e-5 * max(1.0, abs(fd)), (p, fd, out[1 + p])
+    Ps = rng.normal(size=(30, 25))
+    mean, var = e.predict(torch.as_tensor(Ps, device="cuda"))
+    S, _ = cpu.spectral_filter(lam, *th[:3], 900.0)
+    mr, vr = cpu.sgpr_predict(Phi, y, S, th[3], Z, Ps)
+    np.testing.assert_allclose(mean.cpu().numpy(), mr.ravel(), rtol=1e-8, atol=1e-10)
+    np.testing.assert_allclose(var.cpu().numpy(), vr, rtol=1e-8, atol=1e-10)
+
+
+def test_fit_with_inducing_points_vs_oracle():
+    """RVGP.fit(..., n_inducing_points=m): inducing rows from FPS of the
+    training rows (bit-exact vs the oracle's FPS on the same rows); after
+    L-BFGS-B iterations the loss is the oracle's SGPR bound at the fitted
+    hyperparameters and below its starting value; held-out predictions vs the
+    oracle's SGPR predictive equations."""
+    import torch
+    import rvgp_amd as RVGP
+    from tests.util import sphere
+    X = sphere(500, 43)
+    d = RVGP.create_data_object(X, n_neighbors=10, n_eigenpairs=25, verbose=False)
+    d.random_vector_field(seed=3)
+    d.smooth_vector_field(t=10)
+    train = np.random.default_rng(5).choice(500, size=300, replace=False)
+    # m = 15 inducing rows < k = 25 features: Kuu = Z S Z^T is regular (with
+    # m > k it is singular up to GPflow's jitter and the bound is fragile)
+    gp = RVGP.fit(d, train_ind=train, n_inducing_points=15, epochs=10, verbose=False)
+    tr, te = cpu.train_test_split_indices(len(train), 0.2, 0)
+    nodes = train[tr]
+    Phi = d.phi_rows(torch.as_tensor(nodes.astype(np.int32), device="cuda")).cpu().numpy()
+    y = np.asarray(d.vectors)[nodes].reshape(-1, 1)
+    ind, _ = cpu.fps(Phi, N=15)
+    np.testing.assert_array_equal(gp.engine.inducing_rows, ind)
+    Z = Phi[ind]
+    lam = np.asarray(d.evals_Lc)
+    # the positivity bounds the kernel was created with (set_default_positive_
+    # minimum is applied after the kernel exists, main.py:30 vs 52)
+    lower = gp.engine.lower
+
+    def f(u):
+        t = cpu.softplus(u) + lower
+        S, _ = cpu.spectral_filter(lam, t[0], t[1], t[2], 500 * 3)
+        return cpu.sgpr_nlml(Phi, y, S, t[3], Z)
+    u0 = cpu.inv_softplus(np.array([1.5, 5.0, 1.0, 1.0]) - lower)
+    loss = gp.training_loss()
+    assert abs(loss - f(gp.engine.u)) < 1e-8 * abs(loss)
+    assert loss < f(u0)
+    te_nodes = [int(i) for i in np.setdiff1d(np.arange(500), train)[:50]]
+    mean, var = gp.transform(d, te_nodes)
+    th = gp.engine.theta_host()
+    S, _ = cpu.spectral_filter(lam, th[0], th[1], th[2], 500 * 3)
+    Ps = d.phi_rows(torch.as_tensor(np.asarray(te_nodes, np.int32), device="cuda")).cpu().numpy()
+    mr, vr = cpu.sgpr_predict(Phi, y, S, th[3], Z, Ps)
+    assert mean.shape == (50, 3)
+    np.testing.assert_allclose(mean.ravel(), mr.ravel(), rtol=1e-7, atol=1e-9)
+    np.testing.assert_allclose(var.ravel(), vr.ravel(), rtol=1e-7, atol=1e-9)

@@ -327,3 +327,27 @@ def test_rbf_gpr_oracle_vs_multivariate_normal_and_fd():
     m, v = cpu.rbf_gpr_predict(X, Y, 1.0, 1.5, 1e-10, X)
     np.testing.assert_allclose(m, Y, atol=1e-5)
     assert np.abs(v).max() < 1e-5
+
+
+def test_sgpr_oracle_bound_properties():
+    """The SGPR oracle (GPflow's SGPR.elbo formula): (1) with the training
+    inputs themselves as inducing points (N < k, so K_uu is regular) the bound
+    equals the exact GPR log marginal likelihood up to the jitter; (2) with
+    fewer inducing points it is a lower bound (-ELBO >= NLML); (3) the
+    predictive mean with Z = X equals the exact GPR mean."""
+    rng = np.random.default_rng(23)
+    Phi = rng.normal(size=(12, 20))
+    y = rng.normal(size=(12, 1))
+    S = rng.uniform(0.2, 2.0, 20)
+    exact = cpu.gpr_nlml(Phi, y, S, 0.3)
+    assert abs(cpu.sgpr_nlml(Phi, y, S, 0.3, Phi, jitter=1e-12) - exact) < 1e-7 * abs(exact)
+    Phi2 = rng.normal(size=(80, 20))
+    y2 = rng.normal(size=(80, 1))
+    for m in (3, 10, 25):
+        Z = Phi2[rng.choice(80, m, replace=False)]
+        assert cpu.sgpr_nlml(Phi2, y2, S, 0.3, Z) >= cpu.gpr_nlml(Phi2, y2, S, 0.3) - 1e-9
+    Ps = rng.normal(size=(7, 20))
+    m1, v1 = cpu.sgpr_predict(Phi, y, S, 0.3, Phi, Ps, jitter=1e-12)
+    m2, v2 = cpu.gpr_predict(Phi, y, S, 0.3, Ps)
+    np.testing.assert_allclose(m1.ravel(), m2.ravel(), rtol=1e-7, atol=1e-9)
+    np.testing.assert_allclose(v1.ravel(), v2.ravel(), rtol=1e-6, atol=1e-9)

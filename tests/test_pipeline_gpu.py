@@ -81,8 +81,12 @@ def test_fit_reference_quirks(pair):
     d, _, _ = pair
     with pytest.raises(AttributeError):
         RVGP.fit(d, train_ind=np.arange(100), kernel_variance=1.0, epochs=1, verbose=False)
-    with pytest.raises(NotImplementedError):
-        RVGP.fit(d, train_ind=np.arange(100), n_inducing_points=10, epochs=1, verbose=False)
+    # a kernel object leaves `input` unbound in the reference (main.py:29-37)
+    with pytest.raises(UnboundLocalError):
+        RVGP.fit(d, train_ind=np.arange(100), kernel=object(), epochs=1, verbose=False)
+    with pytest.raises(NotImplementedError):  # SGPR with the RBF kernel is not built
+        RVGP.fit(d, train_ind=np.arange(100), kernel='rbf', n_inducing_points=10, epochs=1,
+                 verbose=False)
 
 
 def test_mixed_precision_pipeline_within_1e4(pair):

@@ -77,8 +77,18 @@ int rvgp_fps(const double* x, int n, int D, int n_iter, int start_idx, double sp
  * ------------------------------------------------------------------- */
 
 /* nbr (n, k) int32: the k nearest other points of every point, ordered by
- * (squared distance, index).  Requires 0 < k < n, k <= 64, D <= 64. */
-int rvgp_knn(const double* x, int n, int D, int k, int* nbr, void* stream);
+ * (squared distance, index).  Requires 0 < k < n, k <= 64, D <= 64.
+ * qperm: NULL or a permutation of [0, n) giving the order in which queries
+ * are packed into waves (points close on the manifold should be close in it);
+ * the result does not depend on it.  ws:
+ * rvgp_knn_ws_bytes(n, k) bytes -- per-slice top-k lists of the candidate
+ * slices scanned in parallel, and for n > 8192 a bounding pass's per-point
+ * k-th distance over the first 8192 points, which lets the full scan skip
+ * candidates that cannot end in the top k (the result does not depend on
+ * either). */
+size_t rvgp_knn_ws_bytes(int n, int k);
+int rvgp_knn(const double* x, int n, int D, int k, const int* qperm, int* nbr, void* ws,
+             size_t ws_bytes, void* stream);
 
 /* Workspace bytes for rvgp_graph_symmetrize. */
 size_t rvgp_graph_symmetrize_ws_bytes(int n, int k);

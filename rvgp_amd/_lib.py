@@ -27,7 +27,8 @@ SIGNATURES = {
     "rvgp_last_error": (ctypes.c_char_p, []),
     "rvgp_fps_diam2": (I, [P, I, I, P, P]),
     "rvgp_fps": (I, [P, I, I, I, I, D, I, P, P, P, P, P, P]),
-    "rvgp_knn": (I, [P, I, I, I, P, P]),
+    "rvgp_knn_ws_bytes": (S, [I, I]),
+    "rvgp_knn": (I, [P, I, I, I, P, P, P, S, P]),
     "rvgp_graph_symmetrize_ws_bytes": (S, [I, I]),
     "rvgp_graph_symmetrize": (I, [P, I, I, P, P, P, S, P]),
     "rvgp_geo_nbh_ws_bytes": (S, [I, I]),

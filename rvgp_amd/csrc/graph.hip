@@ -1,18 +1,22 @@
 // a2  kNN graph (RVGP/geometry.py:100-123) and its symmetrised CSR
 // (RVGP/lib/ptu_dijkstra.pyx:84-103).
 //
-// kNN: brute force.  A workgroup owns 64 queries (one per lane) and its four
-// waves split the candidate range into four contiguous slices, so n = 1e5
-// queries still put ~6 waves on every SIMD.  Each wave stages its candidate
-// tile in LDS and reads each coordinate as a broadcast; the running top-k
-// lives in registers as an unrolled sorted list (compare-and-shift network,
-// no dynamic register indexing) with the current k-th distance cached in a
-// register, so a rejected candidate costs one compare.  Coordinates past D
-// are zero in both the query and the tile: fma(0, 0, acc) == acc, so the
-// padded sum is bit-identical to the D-term one.  Within a slice candidates
-// arrive in increasing index order and a strict "<" keeps the lower index
-// among equal distances; the slices are then merged into wave 0 in slice
-// order with the same insertion, which keeps that order across slices.
+// kNN: brute force, exact.  Candidates are split into S contiguous slices;
+// a workgroup scans one slice for 256 queries (one per lane, four waves that
+// share the candidate stream) and writes each query's top-k of the slice; a
+// merge kernel combines the S lists per query in slice order.  Each wave
+// reads a candidate's coordinates with scalar loads (the index is
+// wave-uniform); the running top-k lives in registers as an unrolled sorted
+// list (compare-and-shift network, no dynamic register indexing) whose k-th
+// distance rejects a candidate with one compare, and the partial sum over
+// the first 8 coordinates rejects most candidates before the rest are read.
+// Coordinates past D are zero in both the query and the candidate:
+// fma(0, 0, acc) == acc, so the padded sum is bit-identical to the D-term
+// one.  Within a slice candidates arrive in increasing index order and a
+// strict "<" keeps the lower index among equal distances; the merge keeps
+// that order across slices.  For n > 8192 a first scan + merge over the
+// first 8192 points gives every query a bound on its k-th distance that
+// starts the full scan's lists.
 //
 // Symmetrisation: degree count (atomics) -> exclusive scan -> scatter of
 // forward + reverse edges + self loop -> per-row sort + de-duplicate ->
@@ -22,15 +26,17 @@
 
 namespace rvgp {
 
-constexpr int kKnnSlices = 4;             // waves per workgroup = candidate slices
-constexpr int kKnnThreads = 64 * kKnnSlices;
-constexpr int kKnnTileDoubles = 1024;     // LDS doubles per wave tile
+constexpr int kKnnWaves = 4;              // query waves per workgroup, one candidate stream
+constexpr int kKnnThreads = 64 * kKnnWaves;
+constexpr int kKnnMergeThreads = 256;
+#define KNN_CHUNK 8                       // coordinates between early-exit tests
+#define KNN_GROUP 4                       // candidates whose first chunks load together
 
 // The running list is right-aligned in KMAX slots: slots [0, KMAX-k) hold a
 // -inf sentinel that never moves, slots [KMAX-k, KMAX) hold the k best so far
-// in ascending order (+inf while empty), so the k-th distance is always
-// bd[KMAX-1].  Insert (d, j), d < bd[KMAX-1]: entries <= d stay in front,
-// larger ones shift back by one and the last one drops out.
+// in ascending order, so the k-th distance is always bd[KMAX-1].  Insert
+// (d, j), d < bd[KMAX-1]: entries <= d stay in front, larger ones shift back
+// by one and the last one drops out.
 template <int KMAX>
 __device__ __forceinline__ void knn_insert(double (&bd)[KMAX], int (&bi)[KMAX], double d, int j) {
 #pragma unroll
@@ -46,90 +52,182 @@ __device__ __forceinline__ void knn_insert(double (&bd)[KMAX], int (&bi)[KMAX], 
     }
 }
 
-template <int DMAX, int KMAX>
-__global__ __launch_bounds__(kKnnThreads) void knn_kernel(const double* __restrict__ x, int n,
-                                                          int D, int k, int* __restrict__ nbr) {
-    constexpr int TILE = kKnnTileDoubles / DMAX;  // candidates per tile
-    constexpr int TILE_B = kKnnSlices * kKnnTileDoubles * 8;
-    constexpr int MERGE_B = 64 * KMAX * 12;
-    __shared__ __align__(16) unsigned char lds[TILE_B > MERGE_B ? TILE_B : MERGE_B];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int i = blockIdx.x * 64 + lane;
-    double* tile = reinterpret_cast<double*>(lds) + w * kKnnTileDoubles;
+// Query at position pos: qperm[pos] (a permutation of [0, n)), pos without
+// an order; -1 past n or for an entry out of range, which is then skipped.
+__device__ __forceinline__ int knn_query(const int* __restrict__ qperm, int pos, int n) {
+    if (pos >= n) return -1;
+    const int v = qperm != nullptr ? qperm[pos] : pos;
+    return (unsigned)v < (unsigned)n ? v : -1;
+}
+
+// The k live slots start at +inf, or, given a bound thr_in[i] >= query i's
+// k-th squared distance, at the next double above it: then only candidates
+// with d2 <= thr_in[i] are taken -- every one that can end in the top k, in
+// the same order, so the result is unchanged -- and a slot never filled keeps
+// that value, which no later insertion test lets through.
+template <int KMAX>
+__device__ __forceinline__ void knn_init(double (&bd)[KMAX], int (&bi)[KMAX], int k,
+                                         const double* __restrict__ thr_in, int i, int n) {
+    double live = __longlong_as_double(0x7ff0000000000000LL);
+    if (thr_in != nullptr && i < n && thr_in[i] < live)  // a finite bound
+        live = __longlong_as_double(__double_as_longlong(thr_in[i]) + 1);
+#pragma unroll
+    for (int p = 0; p < KMAX; ++p) {
+        bd[p] = p < KMAX - k ? __longlong_as_double(0xfff0000000000000LL) : live;
+        bi[p] = -1;
+    }
+}
+
+// Scan: workgroup b takes candidate slice b % S and the kKnnWaves * 64
+// query positions of group b / S (one per lane; position pos is query
+// qperm[pos], or pos without a query order).  Its waves read the same candidates
+// in the same order, so a line the scalar cache fetches for one wave serves
+// the others, and consecutive workgroups -- which the dispatcher spreads over
+// the XCDs in turn -- take different slices, so with S = 8 each XCD's L2
+// holds one slice.  Each wave keeps its lanes' top-k of the slice in
+// registers and writes it to lists[(s * k + p) * n + pos].  A query order
+// that puts manifold neighbours in one wave makes the early exit on the
+// first chunk (a wave-wide test) fire for almost every candidate; the
+// result does not depend on it.
+template <int DMAX, int KMAX, bool EXACT>
+__global__ __launch_bounds__(kKnnThreads) __attribute__((amdgpu_waves_per_eu(3))) void knn_scan_kernel(
+    const double* __restrict__ x, int n, int D, int k, int cand_end, int S,
+    const int* __restrict__ qperm, const double* __restrict__ thr_in, double* __restrict__ ld,
+    int* __restrict__ li) {
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int sl = blockIdx.x % S;
+    const int q0 = (blockIdx.x / S * kKnnWaves + w) * 64;
+    if (q0 >= n) return;  // wave-uniform; the kernel has no barrier
+    const int pos = q0 + lane;
+    const int i = knn_query(qperm, pos, n);  // -1: no query in this lane
 
     double q[DMAX];
 #pragma unroll
-    for (int c = 0; c < DMAX; ++c) q[c] = (i < n && c < D) ? x[(size_t)i * D + c] : 0.0;
+    for (int c = 0; c < DMAX; ++c) q[c] = (i >= 0 && c < D) ? x[(size_t)i * D + c] : 0.0;
     double bd[KMAX];
     int bi[KMAX];
-#pragma unroll
-    for (int p = 0; p < KMAX; ++p) {
-        // -inf sentinels in front, +inf (empty) in the k live slots
-        bd[p] = __longlong_as_double(p < KMAX - k ? 0xfff0000000000000LL : 0x7ff0000000000000LL);
-        bi[p] = -1;
-    }
+    knn_init<KMAX>(bd, bi, k, thr_in, i >= 0 ? pos : n, n);
 
-    // Slice w = candidates [s0, s1); every wave runs the same trip count.
-    const int per = ((n + kKnnSlices - 1) / kKnnSlices + TILE - 1) / TILE * TILE;
-    const int s0 = w * per, s1 = min(n, s0 + per);
-    for (int t = 0; t < per; t += TILE) {
-        const int j0 = s0 + t;
-        __syncthreads();
-        for (int e = lane; e < TILE * DMAX; e += 64) {
-            const int jj = e / DMAX, c = e % DMAX, j = j0 + jj;
-            tile[e] = (j < s1 && c < D) ? x[(size_t)j * D + c] : 0.0;
-        }
-        __syncthreads();
-        const int jend = min(TILE, s1 - j0);
-        // two candidates per pass: their DMAX-long FMA chains are independent
-        // and interleave; they are then offered in index order
-        int jj = 0;
-        for (; jj + 1 < jend; jj += 2) {
-            double acc0 = 0.0, acc1 = 0.0;
+    // The candidate index is wave-uniform, so its coordinates are read with
+    // scalar loads into SGPRs that feed the FMAs directly: no LDS staging,
+    // whose broadcast reads (64 lanes x 8 bytes per coordinate) bound an
+    // earlier tiled form.  Coordinates c >= D read as 0 in both the query and
+    // the candidate: fma(0, 0, acc) == acc, so the padded sum is bit-identical
+    // to the D-term one.
+    const int per = (cand_end + S - 1) / S;
+    const int s0 = sl * per, s1 = min(cand_end, s0 + per);
+    constexpr int CH = KNN_CHUNK < DMAX ? KNN_CHUNK : DMAX;
+    // Candidate c's coordinate t, zero past D.  EXACT (D == DMAX) reads a
+    // compile-time-sized contiguous run, so a chunk becomes wide scalar loads
+    // issued together; otherwise the load uses a clamped index and a select
+    // (a branch per coordinate would serialise the loads).
+    auto coord = [&](int c, int t) -> double {
+        if (EXACT) return x[(size_t)c * DMAX + t];
+        const double v = x[(size_t)c * D + (t < D ? t : D - 1)];
+        return t < D ? v : 0.0;
+    };
+    // Finish candidate c's sum from coordinate `from` on, leaving early once
+    // the partial sum reaches every lane's k-th distance (the partial sums
+    // only grow: fma(t, t, acc) >= acc for a finite acc), then offer it.
+    auto finish = [&](int c, double acc, int from) {
 #pragma unroll
-            for (int c = 0; c < DMAX; ++c) {
-                const double t0 = __dsub_rn(q[c], tile[jj * DMAX + c]);
-                const double t1 = __dsub_rn(q[c], tile[(jj + 1) * DMAX + c]);
-                acc0 = __fma_rn(t0, t0, acc0);
-                acc1 = __fma_rn(t1, t1, acc1);
-            }
-            const int j = j0 + jj;
-            if (j != i && acc0 < bd[KMAX - 1]) knn_insert<KMAX>(bd, bi, acc0, j);
-            if (j + 1 != i && acc1 < bd[KMAX - 1]) knn_insert<KMAX>(bd, bi, acc1, j + 1);
-        }
-        if (jj < jend) {
-            const int j = j0 + jj;
-            double acc = 0.0;
+        for (int b = 0; b < DMAX; b += CH) {
+            if (b < from) continue;
+            double v[CH];
 #pragma unroll
-            for (int c = 0; c < DMAX; ++c) {
-                const double tt = __dsub_rn(q[c], tile[jj * DMAX + c]);
-                acc = __fma_rn(tt, tt, acc);
+            for (int u = 0; u < CH; ++u) v[u] = coord(c, b + u);
+#pragma unroll
+            for (int u = 0; u < CH; ++u) {
+                const double t = __dsub_rn(q[b + u], v[u]);
+                acc = __fma_rn(t, t, acc);
             }
-            if (j != i && acc < bd[KMAX - 1]) knn_insert<KMAX>(bd, bi, acc, j);
+            if (b + CH < DMAX && __all(acc >= bd[KMAX - 1])) return;
         }
-    }
+        if (c != i && acc < bd[KMAX - 1]) knn_insert<KMAX>(bd, bi, acc, c);
+    };
 
-    // Merge slices 1..3 into wave 0, in slice order.
-    double* md = reinterpret_cast<double*>(lds);
-    int* mi = reinterpret_cast<int*>(lds + 64 * KMAX * 8);
-    for (int src = 1; src < kKnnSlices; ++src) {
-        __syncthreads();
-        if (w == src) {
+    // Groups of KNN_GROUP candidates: the first chunk of every candidate in
+    // the group is fetched by one batch of scalar loads (one exposed memory
+    // latency per group rather than per candidate), then each candidate
+    // whose first-chunk partial sum is below some lane's k-th distance is
+    // finished, in index order.
+    int j = s0;
+    for (; j + KNN_GROUP <= s1; j += KNN_GROUP) {
+        double v[KNN_GROUP][CH];
 #pragma unroll
-            for (int p = 0; p < KMAX; ++p) {
-                md[p * 64 + lane] = bd[p];
-                mi[p * 64 + lane] = bi[p];
+        for (int g = 0; g < KNN_GROUP; ++g)
+#pragma unroll
+            for (int u = 0; u < CH; ++u) v[g][u] = coord(j + g, u);
+        // keep the group's loads issued back to back, ahead of the arithmetic
+        __builtin_amdgcn_sched_barrier(0);
+        double acc[KNN_GROUP];
+#pragma unroll
+        for (int g = 0; g < KNN_GROUP; ++g) {
+            acc[g] = 0.0;
+#pragma unroll
+            for (int u = 0; u < CH; ++u) {
+                const double t = __dsub_rn(q[u], v[g][u]);
+                acc[g] = __fma_rn(t, t, acc[g]);
             }
         }
-        __syncthreads();
-        if (w == 0) {
-            for (int p = KMAX - k; p < KMAX; ++p) {
-                const double d = md[p * 64 + lane];
-                if (d < bd[KMAX - 1]) knn_insert<KMAX>(bd, bi, d, mi[p * 64 + lane]);
-            }
+        // The group's liveness mask is formed before any candidate is
+        // finished, so the first-chunk values are dead by then (the finish
+        // path reuses their registers).  thr is >= the k-th distance at the
+        // time a later candidate of the group is finished, so a candidate all
+        // lanes reject against thr is rejected then too.
+        const double thr = bd[KMAX - 1];
+        unsigned live = 0;
+#pragma unroll
+        for (int g = 0; g < KNN_GROUP; ++g)
+            if (CH == DMAX || !__all(acc[g] >= thr)) live |= 1u << g;
+        if (live == 0) continue;
+#pragma unroll
+        for (int g = 0; g < KNN_GROUP; ++g)
+            if (live & (1u << g)) finish(j + g, acc[g], CH);
+    }
+    for (; j < s1; ++j) finish(j, 0.0, 0);
+
+    if (i >= 0) {
+#pragma unroll
+        for (int p = 0; p < KMAX; ++p) {
+            if (p < KMAX - k) continue;
+            const size_t e = ((size_t)sl * k + (p - (KMAX - k))) * n + pos;
+            ld[e] = bd[p];
+            li[e] = bi[p];
         }
     }
-    if (w == 0 && i < n) {
+}
+
+// Merge: one thread per query inserts the S slice lists in slice order, each
+// in its own ascending order, with the same strict insertion.  The slices are
+// increasing index ranges, so entries of equal distance arrive in index order
+// and the result is the top k of one scan over all candidates.  A list is
+// left at its first entry that the current k-th distance rejects: the rest
+// of that list is no smaller.
+template <int KMAX>
+__global__ __launch_bounds__(kKnnMergeThreads) void knn_merge_kernel(
+    int n, int k, int S, const int* __restrict__ qperm, const double* __restrict__ thr_in,
+    const double* __restrict__ ld, const int* __restrict__ li, double* __restrict__ thr_out,
+    int* __restrict__ nbr) {
+    const int pos = blockIdx.x * kKnnMergeThreads + threadIdx.x;
+    if (pos >= n) return;
+    const int i = knn_query(qperm, pos, n);
+    if (i < 0) return;
+    double bd[KMAX];
+    int bi[KMAX];
+    knn_init<KMAX>(bd, bi, k, thr_in, pos, n);
+    for (int s = 0; s < S; ++s) {
+        for (int p = 0; p < k; ++p) {
+            const size_t e = ((size_t)s * k + p) * n + pos;
+            const double d = ld[e];
+            if (!(d < bd[KMAX - 1])) break;
+            knn_insert<KMAX>(bd, bi, d, li[e]);
+        }
+    }
+    if (thr_out != nullptr) {
+        thr_out[pos] = bd[KMAX - 1];  // k-th squared distance over the candidates scanned
+    } else {
 #pragma unroll
         for (int p = 0; p < KMAX; ++p)
             if (p >= KMAX - k) nbr[(size_t)i * k + p - (KMAX - k)] = bi[p];
@@ -220,33 +318,88 @@ __global__ void sym_compact_kernel(const int* __restrict__ start, int n,
         if (a == 0 || seg[a] != seg[a - 1]) cols[o++] = seg[a];
 }
 
+template <int DMAX, int KM>
+static void launch_knn_scan(hipStream_t st, const double* x, int n, int D, int k, int cand_end,
+                            int S, const int* qperm, const double* thr_in, double* ld, int* li) {
+    const dim3 g(cdiv(n, 64 * kKnnWaves) * S);
+    if (D == DMAX)
+        hipLaunchKernelGGL((knn_scan_kernel<DMAX, KM, true>), g, dim3(kKnnThreads), 0, st, x, n,
+                           D, k, cand_end, S, qperm, thr_in, ld, li);
+    else
+        hipLaunchKernelGGL((knn_scan_kernel<DMAX, KM, false>), g, dim3(kKnnThreads), 0, st, x, n,
+                           D, k, cand_end, S, qperm, thr_in, ld, li);
+}
+
+template <int KM>
+static void launch_knn_km(hipStream_t st, const double* x, int n, int D, int k, int cand_end,
+                          int S, const int* qperm, const double* thr_in, double* ld, int* li,
+                          double* thr_out, int* nbr) {
+    if (D <= 4) launch_knn_scan<4, KM>(st, x, n, D, k, cand_end, S, qperm, thr_in, ld, li);
+    else if (D <= 8) launch_knn_scan<8, KM>(st, x, n, D, k, cand_end, S, qperm, thr_in, ld, li);
+    else if (D <= 16) launch_knn_scan<16, KM>(st, x, n, D, k, cand_end, S, qperm, thr_in, ld, li);
+    else if (D <= 32) launch_knn_scan<32, KM>(st, x, n, D, k, cand_end, S, qperm, thr_in, ld, li);
+    else launch_knn_scan<64, KM>(st, x, n, D, k, cand_end, S, qperm, thr_in, ld, li);
+    hipLaunchKernelGGL((knn_merge_kernel<KM>), dim3(cdiv(n, kKnnMergeThreads)),
+                       dim3(kKnnMergeThreads), 0, st, n, k, S, qperm, thr_in, ld, li, thr_out,
+                       nbr);
+}
+
+// One scan + merge over candidates [0, cand_end).
+static void launch_knn(hipStream_t st, const double* x, int n, int D, int k, int cand_end, int S,
+                       const int* qperm, const double* thr_in, double* ld, int* li,
+                       double* thr_out, int* nbr) {
+    if (k <= 16) launch_knn_km<16>(st, x, n, D, k, cand_end, S, qperm, thr_in, ld, li, thr_out, nbr);
+    else if (k <= 24) launch_knn_km<24>(st, x, n, D, k, cand_end, S, qperm, thr_in, ld, li, thr_out, nbr);
+    else if (k <= 32) launch_knn_km<32>(st, x, n, D, k, cand_end, S, qperm, thr_in, ld, li, thr_out, nbr);
+    else launch_knn_km<64>(st, x, n, D, k, cand_end, S, qperm, thr_in, ld, li, thr_out, nbr);
+}
+
+// candidates of the bounding pass: the first kKnnBoundCands points
+constexpr int kKnnBoundCands = 8192;
+
+// Candidate slices: 8 (one per XCD) while the query groups alone give few
+// waves; fewer for very large n, where they bound the list workspace.
+static int knn_slices(int n) {
+    return n <= 400000 ? 8 : n <= 800000 ? 4 : n <= 1600000 ? 2 : 1;
+}
+
 }  // namespace rvgp
 
 using namespace rvgp;
 
-template <int DMAX>
-static void launch_knn(dim3 g, hipStream_t s, const double* x, int n, int D, int k, int* nbr) {
-    if (k <= 16)
-        hipLaunchKernelGGL((knn_kernel<DMAX, 16>), g, dim3(kKnnThreads), 0, s, x, n, D, k, nbr);
-    else if (k <= 24)
-        hipLaunchKernelGGL((knn_kernel<DMAX, 24>), g, dim3(kKnnThreads), 0, s, x, n, D, k, nbr);
-    else if (k <= 32)
-        hipLaunchKernelGGL((knn_kernel<DMAX, 32>), g, dim3(kKnnThreads), 0, s, x, n, D, k, nbr);
-    else
-        hipLaunchKernelGGL((knn_kernel<DMAX, 64>), g, dim3(kKnnThreads), 0, s, x, n, D, k, nbr);
+extern "C" size_t rvgp_knn_ws_bytes(int n, int k) {
+    if (n <= 0 || k <= 0) return 0;
+    const size_t lists = (size_t)knn_slices(n) * k * n;
+    const size_t thr = n > kKnnBoundCands ? (size_t)n : 0;
+    return align_up(sizeof(double) * thr, 256) + align_up(sizeof(double) * lists, 256) +
+           sizeof(int) * lists;
 }
 
-extern "C" int rvgp_knn(const double* x, int n, int D, int k, int* nbr, void* stream) {
+extern "C" int rvgp_knn(const double* x, int n, int D, int k, const int* qperm, int* nbr,
+                        void* ws, size_t ws_bytes, void* stream) {
     RVGP_CHECK_ARG(k > 0 && k < n, "rvgp_knn: need 0 < k < n (k=%d, n=%d)", k, n);
     RVGP_CHECK_ARG(k <= 64, "rvgp_knn: k <= 64 supported");
     RVGP_CHECK_ARG(D >= 1 && D <= 64, "rvgp_knn: 1 <= D <= 64 supported");
-    dim3 g(cdiv(n, 64));  // 64 queries per workgroup
-    hipStream_t s = as_stream(stream);
-    if (D <= 4) launch_knn<4>(g, s, x, n, D, k, nbr);
-    else if (D <= 8) launch_knn<8>(g, s, x, n, D, k, nbr);
-    else if (D <= 16) launch_knn<16>(g, s, x, n, D, k, nbr);
-    else if (D <= 32) launch_knn<32>(g, s, x, n, D, k, nbr);
-    else launch_knn<64>(g, s, x, n, D, k, nbr);
+    RVGP_CHECK_ARG(ws != nullptr && ws_bytes >= rvgp_knn_ws_bytes(n, k),
+                   "rvgp_knn: workspace too small");
+    hipStream_t st = as_stream(stream);
+    const int S = knn_slices(n);
+    const size_t lists = (size_t)S * k * n;
+    char* p = static_cast<char*>(ws);
+    double* thr = nullptr;
+    if (n > kKnnBoundCands) {
+        thr = reinterpret_cast<double*>(p);
+        p += align_up(sizeof(double) * (size_t)n, 256);
+    }
+    double* ld = reinterpret_cast<double*>(p);
+    int* li = reinterpret_cast<int*>(p + align_up(sizeof(double) * lists, 256));
+    if (thr != nullptr) {
+        // bounding pass: each query's k-th squared distance among the first
+        // kKnnBoundCands points (>= its k-th over all points)
+        launch_knn(st, x, n, D, k, kKnnBoundCands, S, qperm, nullptr, ld, li, thr, nullptr);
+        RVGP_LAUNCH_CHECK();
+    }
+    launch_knn(st, x, n, D, k, n, S, qperm, thr, ld, li, nullptr, nbr);
     RVGP_LAUNCH_CHECK();
     return RVGP_OK;
 }

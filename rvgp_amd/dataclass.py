@@ -46,7 +46,11 @@ class data:
             print('Fit graph')
         Xd = geometry._dev_f64(vertices)
         N, D = Xd.shape
-        G = geometry.manifold_graph(Xd, n_neighbors=n_neighbors)
+        # Morton renumbering of the leading principal coordinates: groups
+        # manifold neighbours into the kNN's waves and, below, gives the
+        # block SpMM its locality
+        perm = morton_perm(Xd)
+        G = geometry.manifold_graph(Xd, n_neighbors=n_neighbors, query_order=perm)
         if isinstance(vertices, np.ndarray):
             G._positions = vertices
         sync()
@@ -59,7 +63,7 @@ class data:
         # on latency-bound steps (heap walks, small eigenproblems, host round
         # trips).
         kL = N if n_eigenpairs is None else min(int(n_eigenpairs), N)
-        perm = morton_perm(Xd) if reorder else None  # Morton renumbering for SpMM locality
+        perm = perm if reorder else None
         Lb = graph_laplacian(G)
         Lb_e = permute_bsr(Lb, perm) if reorder else Lb
         resL = {}

@@ -61,13 +61,31 @@ def furthest_point_sampling(x, N=None, spacing=0.1, start_idx=0):
 # ---------------------------------------------------------------------------
 # a2  geometry.py:100-123
 # ---------------------------------------------------------------------------
-def knn_device(xd, k):
+def knn_device(xd, k, query_order=None):
+    """nbr (n, k) int32 on the device.  query_order (a permutation of the
+    points, e.g. a Morton order) only sets which queries share a wave: the
+    result does not depend on it."""
     n, D = xd.shape
     if not (0 < k < n):
         raise ValueError(f"Expected n_neighbors < n_samples_fit, but n_neighbors = {k + 1}, "
                          f"n_samples_fit = {n}, n_samples = {n}")
+    # sklearn's check_array rejects NaN / inf; beyond 1e150 the squared
+    # distances themselves would overflow and no neighbour could be ranked
+    amax = float(xd.abs().max().item())
+    if not np.isfinite(amax):
+        raise ValueError("Input X contains NaN or infinity.")
+    if amax >= 1e150:
+        raise ValueError("Input X has coordinates too large for squared distances in float64.")
     nbr = torch.empty((n, k), dtype=torch.int32, device=xd.device)
-    _lib.call("rvgp_knn", _lib.ptr(xd), n, D, int(k), _lib.ptr(nbr), _lib.stream())
+    ws_bytes = _lib.lib().rvgp_knn_ws_bytes(n, int(k))
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=xd.device)
+    qp = None
+    if query_order is not None:
+        qp = torch.as_tensor(query_order, device=xd.device).to(torch.int32).contiguous()
+        if qp.shape != (n,):
+            raise ValueError("query_order must hold one entry per point")
+    _lib.call("rvgp_knn", _lib.ptr(xd), n, D, int(k), _lib.ptr(qp), _lib.ptr(nbr), _lib.ptr(ws),
+              ws_bytes, _lib.stream())
     return nbr
 
 
@@ -84,16 +102,25 @@ def symmetrize_device(nbr):
     return rowptr, cols[:nnz].clone()
 
 
-def manifold_graph(X, typ='knn', n_neighbors=5):
+KNN_ORDER_MIN_N = 4096  # below this the kNN scan is too short for a query order to pay
+
+
+def manifold_graph(X, typ='knn', n_neighbors=5, query_order=None):
     """Fit graph over a pointset X (geometry.py:100-123).
 
     Returns a :class:`SymGraph` -- the symmetrised CSR of kNN + I kept on the
     device; ``.to_networkx()`` gives the networkx graph the reference returns.
+    query_order (default: a Morton order of the leading principal
+    coordinates from n = 4096 on) groups manifold neighbours into the kNN's
+    waves; the graph does not depend on it.
     """
     if typ != 'knn':
         raise NotImplementedError("typ='affinity' (geometry.py:114-118) is not on the GPU path")
     xd = _dev_f64(X)
-    rowptr, cols = symmetrize_device(knn_device(xd, int(n_neighbors)))
+    if query_order is None and xd.shape[0] >= KNN_ORDER_MIN_N:
+        from .reorder import morton_perm
+        query_order = morton_perm(xd)
+    rowptr, cols = symmetrize_device(knn_device(xd, int(n_neighbors), query_order))
     pos = X if isinstance(X, np.ndarray) else None
     return SymGraph(rowptr, cols, xd.shape[0], positions=pos)
 

@@ -34,7 +34,7 @@ def test_arg_validation_reports_value_errors_without_a_gpu():
     """Argument checks run before any device work: null pointers are fine."""
     from rvgp_amd import _lib
     L = _lib.load_symbols_only()
-    rc = L.rvgp_knn(None, 10, 3, 10, None, None)  # k >= n
+    rc = L.rvgp_knn(None, 10, 3, 10, None, None, None, 0, None)  # k >= n
     assert rc == _lib.RVGP_EINVAL
     assert b"0 < k < n" in L.rvgp_last_error()
     rc = L.rvgp_bsr_spmm_f64(10, 4, None, None, None, None, 1, 1, 1.0, 0.0, None, 1, 0.0, None, 1,

@@ -76,7 +76,9 @@ def test_fps_edge_cases(geo):
 
 
 # ---------------------------------------------------------------- a2 kNN + CSR
-@pytest.mark.parametrize("n,D,k", [(2000, 3, 10), (3000, 32, 20), (1000, 8, 15), (500, 3, 40)])
+# n > 8192 takes the bounding pass over the first 8192 points
+@pytest.mark.parametrize("n,D,k", [(2000, 3, 10), (3000, 32, 20), (1000, 8, 15), (500, 3, 40),
+                                   (9000, 3, 10), (10000, 5, 30)])
 def test_knn_bit_exact(geo, n, D, k):
     import torch
     x = np.random.default_rng(7 * n + D).normal(size=(n, D))
@@ -96,6 +98,50 @@ def test_knn_ties_lattice(geo):
     x = g.reshape(-1, 3).astype(np.float64)
     nbr = geo.knn_device(torch.as_tensor(x, device="cuda"), 6).cpu().numpy()
     np.testing.assert_array_equal(nbr, cpu.knn(x, 6))
+
+
+@pytest.mark.parametrize("order", ["morton", "random", "reversed"])
+def test_knn_query_order_does_not_change_result(geo, order):
+    """The query order only packs queries into waves: any permutation gives
+    the oracle's neighbour lists, on both sides of the bounding size."""
+    import torch
+    from rvgp_amd.reorder import morton_perm
+    for n, D, k in [(3000, 6, 12), (9500, 32, 20)]:
+        x = np.random.default_rng(n + D).normal(size=(n, D))
+        xd = torch.as_tensor(x, device="cuda")
+        if order == "morton":
+            perm = morton_perm(xd)
+        elif order == "random":
+            perm = torch.as_tensor(np.random.default_rng(1).permutation(n), device="cuda")
+        else:
+            perm = torch.arange(n - 1, -1, -1, device="cuda")
+        nbr = geo.knn_device(xd, k, query_order=perm).cpu().numpy()
+        np.testing.assert_array_equal(nbr, cpu.knn(x, k))
+
+
+def test_knn_duplicates_above_bounding_size(geo):
+    """9000 points on a 20^3 integer lattice: every point has exact
+    duplicates, so bounds of 0 and long runs of equal distances cross the
+    bounding pass; lower index wins every tie."""
+    import torch
+    x = np.random.default_rng(3).integers(0, 20, size=(9000, 3)).astype(np.float64)
+    for k in (4, 12):
+        nbr = geo.knn_device(torch.as_tensor(x, device="cuda"), k).cpu().numpy()
+        np.testing.assert_array_equal(nbr, cpu.knn(x, k))
+
+
+def test_knn_rejects_non_finite(geo):
+    import torch
+    x = np.random.default_rng(0).normal(size=(50, 3))
+    x[7, 1] = np.nan
+    with pytest.raises(ValueError):
+        geo.knn_device(torch.as_tensor(x, device="cuda"), 5)
+    x[7, 1] = 1e200
+    with pytest.raises(ValueError):
+        geo.knn_device(torch.as_tensor(x, device="cuda"), 5)
+    x[7, 1] = 0.0
+    with pytest.raises(ValueError):  # an order must cover every point
+        geo.knn_device(torch.as_tensor(x, device="cuda"), 5, query_order=np.arange(49))
 
 
 def test_manifold_graph_matches_networkx(geo):

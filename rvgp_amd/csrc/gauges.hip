@@ -23,6 +23,52 @@
 
 namespace rvgp {
 
+// The neighbourhood matrix G (D x Kp1, point q in column q) gathered from x
+// and centred (pyx:397-404): the mean of each coordinate summed in pop order.
+__device__ void gauges_load_centred(const double* __restrict__ x, int D, const int* idx,
+                                    int Kp1, double* G, int lane) {
+    for (int e = lane; e < D * Kp1; e += 64) {
+        int q = e / D, p = e - q * D;
+        G[p * Kp1 + q] = x[(size_t)idx[q] * D + p];
+    }
+    __syncthreads();
+    for (int p = lane; p < D; p += 64) {
+        double mean = 0.0;
+        for (int q = 0; q < Kp1; ++q) mean += G[p * Kp1 + q];
+        mean = mean / Kp1;
+        for (int q = 0; q < Kp1; ++q) G[p * Kp1 + q] -= mean;
+    }
+    __syncthreads();
+}
+
+// LDS plan, in doubles (regions rounded to even counts so the 16-byte
+// rotation records stay aligned):
+//   R1: G while the Gram matrix is formed, then V;
+//   R2: A; in the Gram case G again once the eigenvalues are copied out;
+//   then the rotation records (c, s), the eigenvalues, the pairs (p, q), the
+//   neighbourhood ids and the eigenvalue order.
+// Reusing G's space for V and A's for the second G keeps one wave's
+// footprint at ~17 KiB for D = 32, K + 1 = 31, so nine waves share a CU.
+struct GaugesLds {
+    int m, m2, lda, half, r1, r2;
+    __host__ __device__ GaugesLds(int D, int Kp1) {
+        const bool cov = D <= Kp1;
+        m = cov ? D : Kp1;
+        m2 = m + (m & 1);
+        lda = m2 + 1;
+        half = m2 / 2;
+        const int g = D * Kp1, a = m2 * lda;
+        r1 = g > a ? g : a;
+        r2 = (cov || a >= g) ? a : g;
+        r1 += r1 & 1;
+        r2 += r2 & 1;
+    }
+    __host__ __device__ size_t bytes(int Kp1) const {
+        return sizeof(double) * ((size_t)r1 + r2 + 2 * (size_t)half + m2) +
+               sizeof(int) * (2 * (size_t)half + Kp1 + m);
+    }
+};
+
 __global__ __launch_bounds__(64) void gauges_kernel(const double* __restrict__ x, int n, int D,
                                                     const int* __restrict__ nbh, int Kp1,
                                                     int ncols, int nsv, int n_check,
@@ -33,33 +79,20 @@ __global__ __launch_bounds__(64) void gauges_kernel(const double* __restrict__ x
     const int i = blockIdx.x;
     const int lane = threadIdx.x;
     const bool cov = D <= Kp1;
-    const int m = cov ? D : Kp1;
-    const int m2 = m + (m & 1);
-    double* G = smem;                  // D x Kp1
-    double* A = G + D * Kp1;           // m2 x m2
-    const int lda = m2 + 1;            // padded row stride of A and V
-    double* V = A + m2 * lda;          // m2 x m2
-    double* cs = V + m2 * lda;         // m2/2 cosines
-    double* sn = cs + m2 / 2;          // m2/2 sines
-    int* pp = reinterpret_cast<int*>(sn + m2 / 2);  // m2/2
-    int* qq = pp + m2 / 2;                          // m2/2
-    int* idx = qq + m2 / 2;                         // Kp1
-    int* order = idx + Kp1;                         // m
+    const GaugesLds L(D, Kp1);
+    const int m = L.m, m2 = L.m2, lda = L.lda, half = L.half;
+    double* G = smem;                                        // R1
+    double* V = smem;                                        // R1, once A is formed
+    double* A = smem + L.r1;                                 // R2
+    double2* csn = reinterpret_cast<double2*>(A + L.r2);     // half (c, s)
+    double* lam = reinterpret_cast<double*>(csn + half);     // m2
+    int2* pqv = reinterpret_cast<int2*>(lam + m2);           // half (p, q)
+    int* idx = reinterpret_cast<int*>(pqv + half);           // Kp1
+    int* order = idx + Kp1;                                  // m
 
     for (int q = lane; q < Kp1; q += 64) idx[q] = nbh[(size_t)i * Kp1 + q];
     __syncthreads();
-    for (int e = lane; e < D * Kp1; e += 64) {
-        int q = e / D, p = e - q * D;
-        G[p * Kp1 + q] = x[(size_t)idx[q] * D + p];
-    }
-    __syncthreads();
-    for (int p = lane; p < D; p += 64) {  // pyx:397-404
-        double mean = 0.0;
-        for (int q = 0; q < Kp1; ++q) mean += G[p * Kp1 + q];
-        mean = mean / Kp1;
-        for (int q = 0; q < Kp1; ++q) G[p * Kp1 + q] -= mean;
-    }
-    __syncthreads();
+    gauges_load_centred(x, D, idx, Kp1, G, lane);
     for (int e = lane; e < m2 * m2; e += 64) {
         int a = e / m2, b = e - a * m2;
         double s = 0.0;
@@ -71,11 +104,14 @@ __global__ __launch_bounds__(64) void gauges_kernel(const double* __restrict__ x
             }
         }
         A[a * lda + b] = s;
+    }
+    __syncthreads();  // G is dead: V takes its space
+    for (int e = lane; e < m2 * m2; e += 64) {
+        int a = e / m2, b = e - a * m2;
         V[a * lda + b] = (a == b) ? 1.0 : 0.0;
     }
     __syncthreads();
 
-    const int half = m2 / 2;
     // t / half and t / m2 for t < 4096 by multiply-shift (exact: the rounding
     // error t (ceil(2^20/d) - 2^20/d) / 2^20 < 1/64 <= 1/d, and t * r < 2^32)
     const unsigned rh = ((1u << 20) + half - 1) / half, rm = ((1u << 20) + m2 - 1) / m2;
@@ -109,10 +145,8 @@ __global__ __launch_bounds__(64) void gauges_kernel(const double* __restrict__ x
                     c = 1.0 / sqrt(1.0 + t * t);
                     s = t * c;
                 }
-                pp[lane] = p;
-                qq[lane] = q;
-                cs[lane] = c;
-                sn[lane] = s;
+                pqv[lane] = make_int2(p, q);
+                csn[lane] = make_double2(c, s);
             }
             // the workgroup is this one wavefront: skip a step that rotates
             // nothing (identity rotations)
@@ -124,8 +158,10 @@ __global__ __launch_bounds__(64) void gauges_kernel(const double* __restrict__ x
             // columns of pair k2) and the blocks update independently.
             for (int t = lane; t < half * half; t += 64) {
                 const int k = (int)(((unsigned)t * rh) >> 20), k2 = t - k * half;
-                const int p = pp[k], q = qq[k], p2 = pp[k2], q2 = qq[k2];
-                const double c = cs[k], s = sn[k], c2 = cs[k2], s2 = sn[k2];
+                const int2 r = pqv[k], cc = pqv[k2];
+                const double2 w = csn[k], w2 = csn[k2];
+                const int p = r.x, q = r.y, p2 = cc.x, q2 = cc.y;
+                const double c = w.x, s = w.y, c2 = w2.x, s2 = w2.y;
                 const double a00 = A[p * lda + p2], a01 = A[p * lda + q2];
                 const double a10 = A[q * lda + p2], a11 = A[q * lda + q2];
                 // rows: [p; q] <- [c -s; s c]^T ... applied as J^T on the left
@@ -139,22 +175,24 @@ __global__ __launch_bounds__(64) void gauges_kernel(const double* __restrict__ x
             }
             for (int t = lane; t < half * m2; t += 64) {  // V <- V J
                 const int k = (int)(((unsigned)t * rm) >> 20), r = t - k * m2;
-                const int p = pp[k], q = qq[k];
-                const double c = cs[k], s = sn[k];
-                const double vrp = V[r * lda + p], vrq = V[r * lda + q];
-                V[r * lda + p] = c * vrp - s * vrq;
-                V[r * lda + q] = s * vrp + c * vrq;
+                const int2 pq = pqv[k];
+                const double2 w = csn[k];
+                const double vrp = V[r * lda + pq.x], vrq = V[r * lda + pq.y];
+                V[r * lda + pq.x] = w.x * vrp - w.y * vrq;
+                V[r * lda + pq.y] = w.y * vrp + w.x * vrq;
             }
             __syncthreads();
         }
         if (!any_rot) break;
     }
+    for (int a = lane; a < m; a += 64) lam[a] = A[a * lda + a];
+    __syncthreads();
     // eigenvalues descending (ties: lower index first)
     for (int a = lane; a < m; a += 64) {
-        double la = A[a * lda + a];
+        double la = lam[a];
         int rank = 0;
         for (int b = 0; b < m; ++b) {
-            double lb = A[b * lda + b];
+            double lb = lam[b];
             rank += (lb > la) || (lb == la && b < a);
         }
         order[rank] = a;
@@ -162,11 +200,15 @@ __global__ __launch_bounds__(64) void gauges_kernel(const double* __restrict__ x
     __syncthreads();
     int bad = 0;
     for (int r = lane; r < m; r += 64) {
-        double lam = A[order[r] * lda + order[r]];
-        double s = sqrt(lam > 0.0 ? lam : 0.0);
+        double lv = lam[order[r]];
+        double s = sqrt(lv > 0.0 ? lv : 0.0);
         if (r < nsv) sigma[(size_t)i * nsv + r] = s;
         if (r < n_check && s < 1e-10) bad = 1;
     }
+    // Gram case: the left singular vectors are G v / s, with G rebuilt in
+    // A's space (the same gather and centring, so the same values)
+    double* G2 = A;
+    if (!cov && ncols > 0) gauges_load_centred(x, D, idx, Kp1, G2, lane);
     for (int e = lane; e < D * ncols; e += 64) {
         int p = e / ncols, r = e - p * ncols;
         int c = order[r];
@@ -174,10 +216,10 @@ __global__ __launch_bounds__(64) void gauges_kernel(const double* __restrict__ x
         if (cov) {
             u = V[p * lda + c];
         } else {
-            double lam = A[c * lda + c];
-            double s = sqrt(lam > 0.0 ? lam : 0.0);
+            double lv = lam[c];
+            double s = sqrt(lv > 0.0 ? lv : 0.0);
             double acc = 0.0;
-            for (int q = 0; q < Kp1; ++q) acc = __fma_rn(G[p * Kp1 + q], V[q * lda + c], acc);
+            for (int q = 0; q < Kp1; ++q) acc = __fma_rn(G2[p * Kp1 + q], V[q * lda + c], acc);
             u = s > 0.0 ? acc / s : 0.0;
         }
         tangents[(size_t)i * D * ncols + e] = u;
@@ -185,13 +227,7 @@ __global__ __launch_bounds__(64) void gauges_kernel(const double* __restrict__ x
     if (__any(bad) && lane == 0) atomicMax(status, RVGP_ST_RANK_DEFICIENT);
 }
 
-static size_t gauges_smem(int D, int Kp1) {
-    int m = D <= Kp1 ? D : Kp1;
-    int m2 = m + (m & 1);
-    size_t dbl = (size_t)D * Kp1 + 2 * (size_t)m2 * (m2 + 1) + (size_t)m2;
-    size_t ints = 2 * (size_t)(m2 / 2) + Kp1 + m;
-    return dbl * sizeof(double) + ints * sizeof(int);
-}
+static size_t gauges_smem(int D, int Kp1) { return GaugesLds(D, Kp1).bytes(Kp1); }
 
 }  // namespace rvgp
 

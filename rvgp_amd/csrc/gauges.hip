@@ -25,18 +25,19 @@ namespace rvgp {
 
 // The neighbourhood matrix G (D x Kp1, point q in column q) gathered from x
 // and centred (pyx:397-404): the mean of each coordinate summed in pop order.
+// G[p * rs + q * cs] (rs = Kp1, cs = 1: row-major).
 __device__ void gauges_load_centred(const double* __restrict__ x, int D, const int* idx,
-                                    int Kp1, double* G, int lane) {
+                                    int Kp1, double* G, int lane, int rs, int cs) {
     for (int e = lane; e < D * Kp1; e += 64) {
         int q = e / D, p = e - q * D;
-        G[p * Kp1 + q] = x[(size_t)idx[q] * D + p];
+        G[p * rs + q * cs] = x[(size_t)idx[q] * D + p];
     }
     __syncthreads();
     for (int p = lane; p < D; p += 64) {
         double mean = 0.0;
-        for (int q = 0; q < Kp1; ++q) mean += G[p * Kp1 + q];
+        for (int q = 0; q < Kp1; ++q) mean += G[p * rs + q * cs];
         mean = mean / Kp1;
-        for (int q = 0; q < Kp1; ++q) G[p * Kp1 + q] -= mean;
+        for (int q = 0; q < Kp1; ++q) G[p * rs + q * cs] -= mean;
     }
     __syncthreads();
 }
@@ -92,7 +93,7 @@ __global__ __launch_bounds__(64) void gauges_kernel(const double* __restrict__ x
 
     for (int q = lane; q < Kp1; q += 64) idx[q] = nbh[(size_t)i * Kp1 + q];
     __syncthreads();
-    gauges_load_centred(x, D, idx, Kp1, G, lane);
+    gauges_load_centred(x, D, idx, Kp1, G, lane, Kp1, 1);
     for (int e = lane; e < m2 * m2; e += 64) {
         int a = e / m2, b = e - a * m2;
         double s = 0.0;
@@ -208,7 +209,7 @@ __global__ __launch_bounds__(64) void gauges_kernel(const double* __restrict__ x
     // Gram case: the left singular vectors are G v / s, with G rebuilt in
     // A's space (the same gather and centring, so the same values)
     double* G2 = A;
-    if (!cov && ncols > 0) gauges_load_centred(x, D, idx, Kp1, G2, lane);
+    if (!cov && ncols > 0) gauges_load_centred(x, D, idx, Kp1, G2, lane, Kp1, 1);
     for (int e = lane; e < D * ncols; e += 64) {
         int p = e / ncols, r = e - p * ncols;
         int c = order[r];
@@ -229,6 +230,154 @@ __global__ __launch_bounds__(64) void gauges_kernel(const double* __restrict__ x
 
 static size_t gauges_smem(int D, int Kp1) { return GaugesLds(D, Kp1).bytes(Kp1); }
 
+// Wide ambient space (D > K+1): one-sided (Hestenes) Jacobi on the columns
+// of the centred neighbourhood matrix itself.  Rotating column pairs until
+// every pair is orthogonal leaves G V = U Sigma: the column norms are the
+// singular values and the normalised columns the left singular vectors, so
+// neither the Gram matrix nor V is formed, and small singular values keep
+// their relative accuracy (the Gram matrix squares them).  G is held column
+// by column in LDS; a step rotates the m2/2 disjoint pairs of the
+// round-robin ordering, LP lanes per pair, each lane holding RPL rows of
+// both columns in registers, with the three dot products summed across the
+// pair's lanes by an xor butterfly (identical sums in every lane, so the
+// lanes agree on every rotation).  Padding columns and rows are zero.
+struct OneSidedPlan {
+    int m2, half, lp, rpl, ldg;
+    __host__ __device__ OneSidedPlan(int D, int Kp1) {
+        m2 = Kp1 + (Kp1 & 1);
+        half = m2 / 2;
+        int hp = 1;
+        while (hp < half) hp <<= 1;
+        lp = 64 / hp;                     // lanes per pair (>= 2: K+1 < D <= 64)
+        const int r = (D + lp - 1) / lp;  // rows per lane
+        rpl = r <= 4 ? 4 : r <= 8 ? 8 : r <= 16 ? 16 : 32;
+        // +2 doubles: consecutive columns start 16 bytes x (odd) apart, so
+        // the lanes' 16-byte reads of different columns spread over the banks
+        ldg = lp * rpl + 2;
+    }
+    __host__ __device__ size_t bytes(int Kp1) const {
+        return sizeof(double) * ((size_t)m2 * ldg + m2) + sizeof(int) * ((size_t)Kp1 + m2);
+    }
+};
+
+template <int RPL>
+__global__ __launch_bounds__(64) void gauges_onesided_kernel(
+    const double* __restrict__ x, int n, int D, const int* __restrict__ nbh, int Kp1, int ncols,
+    int nsv, int n_check, double* __restrict__ tangents, double* __restrict__ sigma,
+    int* __restrict__ status) {
+    extern __shared__ double smem[];
+    const int i = blockIdx.x;
+    const int lane = threadIdx.x;
+    const OneSidedPlan P(D, Kp1);
+    const int m2 = P.m2, half = P.half, lp = P.lp, ldg = P.ldg;
+    double* Gc = smem;                                       // column c at Gc + c * ldg
+    double* sig = Gc + (size_t)m2 * ldg;                     // m2 column norms
+    int* idx = reinterpret_cast<int*>(sig + m2);             // Kp1
+    int* order = idx + Kp1;                                  // m2
+
+    for (int e = lane; e < m2 * ldg; e += 64) Gc[e] = 0.0;
+    for (int q = lane; q < Kp1; q += 64) idx[q] = nbh[(size_t)i * Kp1 + q];
+    __syncthreads();
+    gauges_load_centred(x, D, idx, Kp1, Gc, lane, 1, ldg);  // pyx:397-404
+
+    // columns of norm^2 below 1e-28 |G|_F^2 are numerically zero (the
+    // centring leaves at least one): their rotations would only stir noise
+    double fro = 0.0;
+    for (int e = lane; e < m2 * ldg; e += 64) fro = __fma_rn(Gc[e], Gc[e], fro);
+    fro = wave_sum(fro);
+    const double floor2 = 1e-28 * fro;
+
+    const int k = lane / lp, j = lane - k * lp;   // pair, row block
+    const bool act = k < half;
+    const int r0 = j * RPL;
+    for (int sweep = 0; sweep < 40; ++sweep) {
+        bool any_rot = false;
+        for (int step = 0; step < m2 - 1; ++step) {
+            int p = 0, q = 1;
+            if (act) {
+                // round robin: position 0 fixed, positions 1..m2-1 rotate
+                const int a = (k == 0) ? 0 : 1 + (k - 1 + step) % (m2 - 1);
+                const int b = 1 + (m2 - 1 - k - 1 + step) % (m2 - 1);
+                p = a < b ? a : b;
+                q = a < b ? b : a;
+            }
+            double xp[RPL], xq[RPL];
+            double al = 0.0, be = 0.0, ga = 0.0;
+            if (act) {
+#pragma unroll
+                for (int t = 0; t < RPL; t += 2) {
+                    const double2 u = *reinterpret_cast<const double2*>(Gc + p * ldg + r0 + t);
+                    const double2 v = *reinterpret_cast<const double2*>(Gc + q * ldg + r0 + t);
+                    xp[t] = u.x; xp[t + 1] = u.y;
+                    xq[t] = v.x; xq[t + 1] = v.y;
+                }
+#pragma unroll
+                for (int t = 0; t < RPL; ++t) {
+                    al = __fma_rn(xp[t], xp[t], al);
+                    be = __fma_rn(xq[t], xq[t], be);
+                    ga = __fma_rn(xp[t], xq[t], ga);
+                }
+            }
+            for (int o = 1; o < lp; o <<= 1) {
+                al += __shfl_xor(al, o, 64);
+                be += __shfl_xor(be, o, 64);
+                ga += __shfl_xor(ga, o, 64);
+            }
+            const bool rot = act && al > floor2 && be > floor2 &&
+                             fabs(ga) > 1e-14 * sqrt(al * be);
+            if (!__any(rot)) continue;
+            any_rot = true;
+            if (rot) {
+                // [p q] <- [p q] [c s; -s c] with p.q = 0 afterwards
+                const double zeta = (be - al) / (2.0 * ga);
+                double t;
+                if (fabs(zeta) > 1e150) t = 0.5 / zeta;
+                else t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+                const double c = 1.0 / sqrt(1.0 + t * t), s = t * c;
+#pragma unroll
+                for (int u = 0; u < RPL; u += 2) {
+                    const double2 np = make_double2(c * xp[u] - s * xq[u], c * xp[u + 1] - s * xq[u + 1]);
+                    const double2 nq = make_double2(s * xp[u] + c * xq[u], s * xp[u + 1] + c * xq[u + 1]);
+                    *reinterpret_cast<double2*>(Gc + p * ldg + r0 + u) = np;
+                    *reinterpret_cast<double2*>(Gc + q * ldg + r0 + u) = nq;
+                }
+            }
+            __syncthreads();  // the next step pairs columns written by other lanes
+        }
+        if (!any_rot) break;
+    }
+    // singular values: column norms; descending, ties lower index first
+    for (int c = lane; c < m2; c += 64) {
+        double s2 = 0.0;
+        for (int r = 0; r < D; ++r) s2 = __fma_rn(Gc[c * ldg + r], Gc[c * ldg + r], s2);
+        sig[c] = sqrt(s2);
+    }
+    __syncthreads();
+    for (int a = lane; a < Kp1; a += 64) {
+        const double la = sig[a];
+        int rank = 0;
+        for (int b = 0; b < Kp1; ++b) {
+            const double lb = sig[b];
+            rank += (lb > la) || (lb == la && b < a);
+        }
+        order[rank] = a;
+    }
+    __syncthreads();
+    int bad = 0;
+    for (int r = lane; r < Kp1; r += 64) {
+        const double sv = sig[order[r]];
+        if (r < nsv) sigma[(size_t)i * nsv + r] = sv;
+        if (r < n_check && sv < 1e-10) bad = 1;
+    }
+    for (int e = lane; e < D * ncols; e += 64) {
+        const int p = e / ncols, r = e - p * ncols;
+        const int c = order[r];
+        const double sv = sig[c];
+        tangents[(size_t)i * D * ncols + e] = sv > 0.0 ? Gc[c * ldg + p] / sv : 0.0;
+    }
+    if (__any(bad) && lane == 0) atomicMax(status, RVGP_ST_RANK_DEFICIENT);
+}
+
 }  // namespace rvgp
 
 using namespace rvgp;
@@ -241,9 +390,25 @@ extern "C" int rvgp_gauges(const double* x, int n, int D, const int* nbh, int Kp
     const int m = D <= Kp1 ? D : Kp1;
     RVGP_CHECK_ARG(ncols >= 0 && ncols <= m && nsv >= 0 && nsv <= m && n_check <= m,
                    "rvgp_gauges: ncols/nsv/n_check must not exceed min(D, K+1)");
+    hipStream_t st = as_stream(stream);
+    if (D > Kp1) {
+        const OneSidedPlan P(D, Kp1);
+        const size_t sm = P.bytes(Kp1);
+        RVGP_CHECK_ARG(sm <= 65536, "rvgp_gauges: D=%d, K+1=%d needs %zu bytes of LDS", D, Kp1, sm);
+#define RVGP_ONESIDED(R)                                                                     \
+    hipLaunchKernelGGL(gauges_onesided_kernel<R>, dim3(n), dim3(64), sm, st, x, n, D, nbh, Kp1, \
+                       ncols, nsv, n_check, tangents, sigma, status)
+        if (P.rpl == 4) RVGP_ONESIDED(4);
+        else if (P.rpl == 8) RVGP_ONESIDED(8);
+        else if (P.rpl == 16) RVGP_ONESIDED(16);
+        else RVGP_ONESIDED(32);
+#undef RVGP_ONESIDED
+        RVGP_LAUNCH_CHECK();
+        return RVGP_OK;
+    }
     const size_t sm = gauges_smem(D, Kp1);
     RVGP_CHECK_ARG(sm <= 65536, "rvgp_gauges: D=%d, K+1=%d needs %zu bytes of LDS (> 64 KiB)", D, Kp1, sm);
-    hipLaunchKernelGGL(gauges_kernel, dim3(n), dim3(64), sm, as_stream(stream), x, n, D, nbh, Kp1,
+    hipLaunchKernelGGL(gauges_kernel, dim3(n), dim3(64), sm, st, x, n, D, nbh, Kp1,
                        ncols, nsv, n_check, tangents, sigma, status);
     RVGP_LAUNCH_CHECK();
     return RVGP_OK;

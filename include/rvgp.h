@@ -104,9 +104,11 @@ int rvgp_graph_symmetrize(const int* nbr, int n, int k, int* rowptr, int* cols, 
  *     ptu_dijkstra.tangent_frames (RVGP/lib/ptu_dijkstra.pyx:33-125, 300-437).
  * ------------------------------------------------------------------- */
 
-/* Workspace bytes for rvgp_geo_nbh with `batch` sources in flight and a
- * node pool of `pool` slots per source (pool >= 1 + K * max row length). */
-size_t rvgp_geo_nbh_ws_bytes(int batch, int pool);
+/* Workspace bytes for rvgp_geo_nbh on n sources with `batch` in flight and
+ * a worst-case node pool of `pool` slots per source (pool >= 1 + K * max row
+ * length).  A first pass gives each source a 512-node pool; the sources that
+ * outgrow it are listed and redone with `pool` (same result). */
+size_t rvgp_geo_nbh_ws_bytes(int n, int batch, int pool);
 
 /* Truncated Dijkstra driven by the Fibonacci heap of ptu_dijkstra.pyx:
  * 440-690, one source per lane: nbh (n, K+1) int32 in pop order

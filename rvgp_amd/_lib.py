@@ -31,7 +31,7 @@ SIGNATURES = {
     "rvgp_knn": (I, [P, I, I, I, P, P, P, S, P]),
     "rvgp_graph_symmetrize_ws_bytes": (S, [I, I]),
     "rvgp_graph_symmetrize": (I, [P, I, I, P, P, P, S, P]),
-    "rvgp_geo_nbh_ws_bytes": (S, [I, I]),
+    "rvgp_geo_nbh_ws_bytes": (S, [I, I, I]),
     "rvgp_geo_nbh": (I, [P, P, P, I, I, I, I, P, P, S, P, P]),
     "rvgp_gauges": (I, [P, I, I, P, I, I, I, I, P, P, P, P]),
     "rvgp_connections": (I, [P, I, I, I, I, P, P, I, P, P, P, P, P, P]),

@@ -164,9 +164,14 @@ __global__ __launch_bounds__(256) void geo_nbh_kernel(const int* __restrict__ ro
                                                       int batch, int pool, int hsize,
                                                       size_t lane_bytes, int* __restrict__ nbh,
                                                       char* __restrict__ ws,
+                                                      const int* __restrict__ src_list,
+                                                      int* __restrict__ ovf_list,
                                                       int* __restrict__ status) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= batch || t >= n) return;
+    // sources: all n, or src_list[1 .. src_list[0]] (the overflow list of a
+    // pass with a smaller pool)
+    const int nsrc = src_list != nullptr ? src_list[0] : n;
+    if (t >= batch || t >= nsrc) return;
     char* base = ws + (size_t)t * lane_bytes;
     HNode* N = reinterpret_cast<HNode*>(base);
     int* keys = reinterpret_cast<int*>(N + pool);
@@ -176,7 +181,8 @@ __global__ __launch_bounds__(256) void geo_nbh_kernel(const int* __restrict__ ro
     for (int q = 0; q < hsize; ++q) keys[q] = -1;
     const int Kp1 = K + 1;
 
-    for (int src = t; src < n; src += batch) {
+    for (int si = t; si < nsrc; si += batch) {
+        const int src = src_list != nullptr ? src_list[1 + si] : si;
         int used = 0;
         bool ok = true;
         auto new_node = [&](int g, unsigned hp) -> int {
@@ -235,6 +241,11 @@ __global__ __launch_bounds__(256) void geo_nbh_kernel(const int* __restrict__ ro
         }
         for (int s = 0; s < used; ++s) keys[N[s].hpos] = -1;
         if (!ok) {
+            if (ovf_list != nullptr) {
+                // redone by the pass with the worst-case pool
+                ovf_list[1 + atomicAdd(&ovf_list[0], 1)] = src;
+                continue;
+            }
             atomicMax(status, RVGP_ST_POOL_OVERFLOW);
             return;
         }
@@ -254,12 +265,43 @@ static size_t lane_bytes_for(int pool) {
     return (b + 255) & ~(size_t)255;
 }
 
+__global__ void nbh_clear_count_kernel(int* __restrict__ c) {
+    if (threadIdx.x == 0) c[0] = 0;
+}
+
+// First pass: every source with a pool of kNbhPool nodes (a search on a
+// kNN graph touches a few hundred); the few sources whose search outgrows it
+// are listed and redone with the worst-case pool 1 + K * max_degree by
+// kNbhBigLanes lanes.  Both passes run the same heap operations, so which
+// pass finishes a source does not change its neighbourhood; the small pool
+// keeps each lane's state at ~29 KB instead of ~80 KB, which is what the
+// latency-bound heap walks spend their time fetching.
+constexpr int kNbhPool = 512;
+constexpr int kNbhBigLanes = 2048;
+
+static void nbh_plan(int n, int batch, int pool, int& p1, int& b2, size_t& o1, size_t& o2,
+                     size_t& o3, size_t& total) {
+    p1 = pool < kNbhPool ? pool : kNbhPool;
+    b2 = batch < kNbhBigLanes ? batch : kNbhBigLanes;
+    const size_t a = lane_bytes_for(p1) * (size_t)batch;
+    const size_t b = p1 < pool ? lane_bytes_for(pool) * (size_t)b2 : 0;
+    const size_t c = p1 < pool ? sizeof(int) * ((size_t)n + 1) : 0;
+    o1 = 0;
+    o2 = a;
+    o3 = a + b;
+    total = a + b + ((c + 255) & ~(size_t)255);
+}
+
 }  // namespace rvgp
 
 using namespace rvgp;
 
-extern "C" size_t rvgp_geo_nbh_ws_bytes(int batch, int pool) {
-    return lane_bytes_for(pool) * (size_t)batch;
+extern "C" size_t rvgp_geo_nbh_ws_bytes(int n, int batch, int pool) {
+    if (n < 1 || batch < 1 || pool < 1) return 0;
+    int p1, b2;
+    size_t o1, o2, o3, total;
+    nbh_plan(n, batch, pool, p1, b2, o1, o2, o3, total);
+    return total;
 }
 
 extern "C" int rvgp_geo_nbh(const int* rowptr, const int* cols, const double* w, int n, int K,
@@ -268,12 +310,29 @@ extern "C" int rvgp_geo_nbh(const int* rowptr, const int* cols, const double* w,
     RVGP_CHECK_ARG(n >= 1 && K >= 0, "rvgp_geo_nbh: need n >= 1, K >= 0");
     RVGP_CHECK_ARG(K < n, "Geodesic neighborhood size must be less than the total number of samples");
     RVGP_CHECK_ARG(batch >= 1 && pool >= K + 1, "rvgp_geo_nbh: bad batch/pool");
-    RVGP_CHECK_ARG(ws_bytes >= rvgp_geo_nbh_ws_bytes(batch, pool), "rvgp_geo_nbh: workspace too small");
-    const int hsize = next_pow2(2 * pool);
+    RVGP_CHECK_ARG(ws_bytes >= rvgp_geo_nbh_ws_bytes(n, batch, pool),
+                   "rvgp_geo_nbh: workspace too small");
+    int p1, b2;
+    size_t o1, o2, o3, total;
+    nbh_plan(n, batch, pool, p1, b2, o1, o2, o3, total);
+    char* base = static_cast<char*>(ws);
+    hipStream_t st = as_stream(stream);
     const int lanes = batch < n ? batch : n;
-    hipLaunchKernelGGL(geo_nbh_kernel, dim3(cdiv(lanes, 256)), dim3(256), 0, as_stream(stream),
-                       rowptr, cols, w, n, K, lanes, pool, hsize, lane_bytes_for(pool), nbh,
-                       static_cast<char*>(ws), status);
+    int* ovf = p1 < pool ? reinterpret_cast<int*>(base + o3) : nullptr;
+    if (ovf != nullptr) {
+        hipLaunchKernelGGL(nbh_clear_count_kernel, dim3(1), dim3(64), 0, st, ovf);
+        RVGP_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(geo_nbh_kernel, dim3(cdiv(lanes, 256)), dim3(256), 0, st, rowptr, cols, w,
+                       n, K, lanes, p1, next_pow2(2 * p1), lane_bytes_for(p1), nbh, base + o1,
+                       nullptr, ovf, status);
     RVGP_LAUNCH_CHECK();
+    if (ovf != nullptr) {
+        const int l2 = b2 < n ? b2 : n;
+        hipLaunchKernelGGL(geo_nbh_kernel, dim3(cdiv(l2, 256)), dim3(256), 0, st, rowptr, cols, w,
+                           n, K, l2, pool, next_pow2(2 * pool), lane_bytes_for(pool), nbh,
+                           base + o2, ovf, nullptr, status);
+        RVGP_LAUNCH_CHECK();
+    }
     return RVGP_OK;
 }

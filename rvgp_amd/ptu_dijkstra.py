@@ -31,15 +31,17 @@ def geo_nbh_device(g, K, max_ws_bytes=16 << 30):
     """(n, K+1) int32 neighbourhoods in heap pop order (pyx:361-394).
 
     One lane per source and every source in flight at once where the
-    workspace cap allows (80 KB per lane at K = 30 and 39-entry rows: 8 GB
-    at n = 1e5): the heap walks are latency-bound, so the number of lanes in
-    flight sets the speed (2 GiB, a quarter of the sources per pass, took
-    twice as long)."""
+    workspace cap allows (~29 KB per lane in the first pass, whose node pool
+    holds what a search on a kNN graph touches; the few searches that
+    outgrow it are redone with the worst-case pool 1 + K * max_degree): the
+    heap walks are latency-bound, so the number of lanes in flight sets the
+    speed."""
     n = g.n
     pool = 1 + K * g.max_row()
-    per_lane = _lib.lib().rvgp_geo_nbh_ws_bytes(1, pool)
+    L = _lib.lib()
+    per_lane = max(1, L.rvgp_geo_nbh_ws_bytes(n, 2, pool) - L.rvgp_geo_nbh_ws_bytes(n, 1, pool))
     batch = int(max(1, min(n, max_ws_bytes // per_lane, 1 << 17)))
-    ws_bytes = _lib.lib().rvgp_geo_nbh_ws_bytes(batch, pool)
+    ws_bytes = L.rvgp_geo_nbh_ws_bytes(n, batch, pool)
     dev = g.cols.device
     ws = torch.empty(ws_bytes, dtype=torch.uint8, device=dev)
     nbh = torch.empty((n, K + 1), dtype=torch.int32, device=dev)

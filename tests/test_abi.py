@@ -50,7 +50,10 @@ def test_workspace_queries():
     from rvgp_amd import _lib
     L = _lib.load_symbols_only()
     assert L.rvgp_graph_symmetrize_ws_bytes(100, 10) >= 4 * 100 * 21
-    assert L.rvgp_geo_nbh_ws_bytes(2, 100) == 2 * L.rvgp_geo_nbh_ws_bytes(1, 100)
+    # pools up to 512 nodes: one pass, workspace linear in the lanes in flight
+    assert L.rvgp_geo_nbh_ws_bytes(1000, 2, 100) == 2 * L.rvgp_geo_nbh_ws_bytes(1000, 1, 100)
+    # a larger worst-case pool adds the second pass's lanes and the overflow list
+    assert L.rvgp_geo_nbh_ws_bytes(1000, 4096, 1200) > 4096 * L.rvgp_geo_nbh_ws_bytes(1000, 1, 512)
     assert L.rvgp_colsumsq_ws_bytes(1000, 10) >= 8 * 10
     assert L.rvgp_colsumsq_ws_bytes(10**6, 10) == 8 * 10 * 1024  # at most 1024 row slices
 

@@ -186,6 +186,29 @@ def test_heap_neighbourhoods_and_gauges_vs_reference(case):
     assert np.abs(I - np.eye(D)).max() < 1e-10
 
 
+def test_heap_neighbourhoods_pool_overflow_second_pass():
+    """A hub joined to 700 nodes: every search that pops it touches more
+    nodes than the first pass's 512-node pool, so those sources are redone
+    by the worst-case-pool pass; all neighbourhoods equal the oracle's."""
+    import scipy.sparse as sp
+    import torch
+    from rvgp_amd import ptu_dijkstra
+    from rvgp_amd.graph import SymGraph
+    n, K = 2000, 30
+    X = np.random.default_rng(11).normal(size=(n, 3))
+    rp0, c0 = cpu.sym_csr(cpu.knn(X, 6))
+    A = sp.csr_matrix((np.ones(len(c0)), c0, rp0), shape=(n, n))
+    hub = sp.coo_matrix((np.ones(700), (np.zeros(700, int), np.arange(1, 701))), shape=(n, n))
+    A = ((A + hub + hub.T) > 0).astype(np.float64).tocsr()
+    A.sort_indices()
+    rp, c = A.indptr.astype(np.int32), A.indices.astype(np.int32)
+    g = SymGraph(torch.as_tensor(rp, device="cuda"), torch.as_tensor(c, device="cuda"), n)
+    nbh = ptu_dijkstra.geo_nbh_device(g, K).cpu().numpy()
+    nbh_ref, _ = cpu.geo_nbh(X, rp, c, K, centred=False)
+    np.testing.assert_array_equal(nbh, nbh_ref)
+    assert (nbh == 0).any(axis=1).sum() > 700  # the hub is in many neighbourhoods
+
+
 def test_tangent_frames_dropin_api(golden_dir):
     from rvgp_amd import ptu_dijkstra, geometry
     z = np.load(os.path.join(golden_dir, "heap_sphere2000_k10.npz"))

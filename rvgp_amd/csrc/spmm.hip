@@ -374,8 +374,13 @@ static int spmm_dispatch(int nb, int b, const int* rowptr, const int* cols, cons
     } else if (vec_ok<T>(r, X, ldx, Y, ldy, Z, ldz, acc, ldacc)) {
         constexpr int VW = 16 / sizeof(T);
         const int nv = r / VW;                        // 16-byte vectors per row
-        const int groups = (int)cdiv(nv, 64);         // column groups of <= 64 vectors
-        const int gv = (nv + groups - 1) / groups;    // balanced group width
+        // column groups of <= 64 vectors, balanced, each a whole number of
+        // 128-byte lines (a multiple of 8 vectors) so no group starts or ends
+        // inside a cache line of a gathered X row
+        int groups = (int)cdiv(nv, 64);
+        int gv = (nv + groups - 1) / groups;
+        gv = gv > 8 ? (gv + 7) / 8 * 8 : gv;
+        groups = (int)cdiv(nv, gv);
         dim3 g(8 * cdiv(cdiv(nb, 4), 8), groups);
         const size_t xbytes = (size_t)nb * b * ldx * sizeof(T);
         if (xbytes < ((size_t)1 << 31)) {

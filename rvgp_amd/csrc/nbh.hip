@@ -23,10 +23,16 @@ struct HNode {
     int gidx, rank, state, parent, left, right, children, hpos;
 };
 
+// The root list is the sibling list holding min_node.  Its two ends are kept
+// in head / tail, so appending a root and finding the leftmost root are O(1)
+// instead of walks over the list: the reference walks it (pyx:526-536,
+// 642-660), but the walks only locate its ends, and the list itself -- every
+// link, every order -- is the same.
 struct Heap {
     HNode* N;
     int* roots;
     int min_node;
+    int head, tail;
 };
 
 __device__ __forceinline__ int rightmost_sibling(HNode* N, int a) {
@@ -60,9 +66,13 @@ __device__ __forceinline__ void add_child(HNode* N, int a, int c) {  // pyx:507-
     }
 }
 
-__device__ __forceinline__ void node_remove(HNode* N, int a) {  // pyx:542-560
+__device__ __forceinline__ void node_remove(Heap& h, int a) {  // pyx:542-560
+    HNode* N = h.N;
     int p = N[a].parent;
-    if (p != -1) {
+    if (p == -1) {  // a root: keep the list's ends
+        if (a == h.head) h.head = N[a].right;
+        if (a == h.tail) h.tail = N[a].left;
+    } else {
         N[p].rank -= 1;
         if (N[a].left != -1) N[p].children = N[a].left;
         else if (N[a].right != -1) N[p].children = N[a].right;
@@ -75,12 +85,24 @@ __device__ __forceinline__ void node_remove(HNode* N, int a) {  // pyx:542-560
     N[a].parent = -1;
 }
 
+// add_sibling(min_node, s) for the root list (pyx:526-536): s goes after
+// the rightmost root; a root's parent is -1, so no rank changes.
+__device__ __forceinline__ void root_append(Heap& h, int s) {
+    HNode* N = h.N;
+    N[h.tail].right = s;
+    N[s].left = h.tail;
+    N[s].right = -1;
+    N[s].parent = -1;
+    h.tail = s;
+}
+
 __device__ __forceinline__ void insert_node(Heap& h, int a) {  // pyx:579-589
     if (h.min_node != -1) {
-        add_sibling(h.N, h.min_node, a);
+        root_append(h, a);
         if (h.N[a].val < h.N[h.min_node].val) h.min_node = a;
     } else {
         h.min_node = a;
+        h.head = h.tail = a;
     }
 }
 
@@ -88,7 +110,7 @@ __device__ __forceinline__ void decrease_val(Heap& h, int a, double v) {  // pyx
     HNode* N = h.N;
     N[a].val = v;
     if (N[a].parent != -1 && N[N[a].parent].val >= v) {
-        node_remove(N, a);
+        node_remove(h, a);
         insert_node(h, a);
     } else if (N[h.min_node].val > N[a].val) {
         h.min_node = a;
@@ -108,10 +130,10 @@ __device__ __forceinline__ bool heap_link(Heap& h, int a) {
         int ln = h.roots[r];
         h.roots[r] = -1;
         if (N[a].val < N[ln].val || a == h.min_node) {
-            node_remove(N, ln);
+            node_remove(h, ln);
             add_child(N, a, ln);
         } else {
-            node_remove(N, a);
+            node_remove(h, a);
             add_child(N, ln, a);
             a = ln;
         }
@@ -127,22 +149,23 @@ __device__ __forceinline__ int remove_min(Heap& h, bool& ok) {
         t = leftmost_sibling(N, N[m].children);
         while (t != -1) {
             tr = N[t].right;
-            node_remove(N, t);
-            add_sibling(N, m, t);
+            node_remove(h, t);
+            root_append(h, t);  // add_sibling(m, t): m is a root
             t = tr;
         }
         N[m].children = -1;
     }
-    t = leftmost_sibling(N, m);
+    t = h.head;  // leftmost_sibling(m)
     if (t == m) {
         if (N[m].right != -1) {
             t = N[m].right;
         } else {
             h.min_node = -1;
+            h.head = h.tail = -1;
             return m;
         }
     }
-    node_remove(N, m);
+    node_remove(h, m);
     h.min_node = t;
     for (int i = 0; i < kMaxRank; ++i) h.roots[i] = -1;
     while (t != -1) {
@@ -198,7 +221,7 @@ __global__ __launch_bounds__(256) void geo_nbh_kernel(const int* __restrict__ ro
             slots[hp] = s;
             return s;
         };
-        Heap h{N, roots, -1};
+        Heap h{N, roots, -1, -1, -1};
         {
             unsigned hp = hash_of(src, mask);
             while (keys[hp] != -1) hp = (hp + 1) & mask;

@@ -18,9 +18,11 @@ namespace rvgp {
 enum : int { kScanned = 0, kNotInHeap = 1, kInHeap = 2 };
 constexpr int kMaxRank = 32;
 
-struct HNode {
+// 32 bytes, aligned: a node never straddles two cache lines
+struct __align__(32) HNode {
     double val;
-    int gidx, rank, state, parent, left, right, children, hpos;
+    int gidx, parent, left, right, children;
+    short rank, state;
 };
 
 // The root list is the sibling list holding min_node.  Its two ends are kept
@@ -197,11 +199,12 @@ __global__ __launch_bounds__(256) void geo_nbh_kernel(const int* __restrict__ ro
     if (t >= batch || t >= nsrc) return;
     char* base = ws + (size_t)t * lane_bytes;
     HNode* N = reinterpret_cast<HNode*>(base);
-    int* keys = reinterpret_cast<int*>(N + pool);
-    int* slots = keys + hsize;
-    int* roots = slots + hsize;
+    // hash table entries (node id, pool slot) side by side: one probe, one line
+    int2* tab = reinterpret_cast<int2*>(N + pool);
+    int* hposv = reinterpret_cast<int*>(tab + hsize);  // table position of each slot
+    int* roots = hposv + pool;
     const unsigned mask = (unsigned)hsize - 1u;
-    for (int q = 0; q < hsize; ++q) keys[q] = -1;
+    for (int q = 0; q < hsize; ++q) tab[q] = make_int2(-1, -1);
     const int Kp1 = K + 1;
 
     for (int si = t; si < nsrc; si += batch) {
@@ -216,15 +219,14 @@ __global__ __launch_bounds__(256) void geo_nbh_kernel(const int* __restrict__ ro
             nd.rank = 0;
             nd.state = kNotInHeap;
             nd.parent = nd.left = nd.right = nd.children = -1;
-            nd.hpos = (int)hp;
-            keys[hp] = g;
-            slots[hp] = s;
+            hposv[s] = (int)hp;
+            tab[hp] = make_int2(g, s);
             return s;
         };
         Heap h{N, roots, -1, -1, -1};
         {
             unsigned hp = hash_of(src, mask);
-            while (keys[hp] != -1) hp = (hp + 1) & mask;
+            while (tab[hp].x != -1) hp = (hp + 1) & mask;
             insert_node(h, new_node(src, hp));
         }
         int scanned = 0;
@@ -241,8 +243,8 @@ __global__ __launch_bounds__(256) void geo_nbh_kernel(const int* __restrict__ ro
                     const int c = cols[e];
                     unsigned hp = hash_of(c, mask);
                     int sc = -1;
-                    while (keys[hp] != -1) {
-                        if (keys[hp] == c) { sc = slots[hp]; break; }
+                    for (int2 te = tab[hp]; te.x != -1; te = tab[hp]) {
+                        if (te.x == c) { sc = te.y; break; }
                         hp = (hp + 1) & mask;
                     }
                     if (sc != -1 && N[sc].state == kScanned) continue;
@@ -262,7 +264,7 @@ __global__ __launch_bounds__(256) void geo_nbh_kernel(const int* __restrict__ ro
                 if (!ok) break;
             }
         }
-        for (int s = 0; s < used; ++s) keys[N[s].hpos] = -1;
+        for (int s = 0; s < used; ++s) tab[hposv[s]] = make_int2(-1, -1);
         if (!ok) {
             if (ovf_list != nullptr) {
                 // redone by the pass with the worst-case pool
@@ -284,7 +286,8 @@ static int next_pow2(int v) {
 
 static size_t lane_bytes_for(int pool) {
     const int hsize = next_pow2(2 * pool);
-    size_t b = sizeof(HNode) * (size_t)pool + sizeof(int) * (2 * (size_t)hsize + kMaxRank);
+    size_t b = sizeof(HNode) * (size_t)pool + sizeof(int2) * (size_t)hsize +
+               sizeof(int) * ((size_t)pool + kMaxRank);
     return (b + 255) & ~(size_t)255;
 }
 
@@ -297,7 +300,7 @@ __global__ void nbh_clear_count_kernel(int* __restrict__ c) {
 // are listed and redone with the worst-case pool 1 + K * max_degree by
 // kNbhBigLanes lanes.  Both passes run the same heap operations, so which
 // pass finishes a source does not change its neighbourhood; the small pool
-// keeps each lane's state at ~29 KB instead of ~80 KB, which is what the
+// keeps each lane's state at ~27 KB instead of ~80 KB, which is what the
 // latency-bound heap walks spend their time fetching.
 constexpr int kNbhPool = 512;
 constexpr int kNbhBigLanes = 2048;

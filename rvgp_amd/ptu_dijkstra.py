@@ -31,7 +31,7 @@ def geo_nbh_device(g, K, max_ws_bytes=16 << 30):
     """(n, K+1) int32 neighbourhoods in heap pop order (pyx:361-394).
 
     One lane per source and every source in flight at once where the
-    workspace cap allows (~29 KB per lane in the first pass, whose node pool
+    workspace cap allows (~27 KB per lane in the first pass, whose node pool
     holds what a search on a kNN graph touches; the few searches that
     outgrow it are redone with the worst-case pool 1 + K * max_degree): the
     heap walks are latency-bound, so the number of lanes in flight sets the

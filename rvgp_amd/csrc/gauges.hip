@@ -292,6 +292,7 @@ __global__ __launch_bounds__(64) void gauges_onesided_kernel(
     const int r0 = j * RPL;
     for (int sweep = 0; sweep < 40; ++sweep) {
         bool any_rot = false;
+        double worst = 0.0;  // largest |cos| between a rotated pair in this sweep
         for (int step = 0; step < m2 - 1; ++step) {
             int p = 0, q = 1;
             if (act) {
@@ -323,11 +324,12 @@ __global__ __launch_bounds__(64) void gauges_onesided_kernel(
                 be += __shfl_xor(be, o, 64);
                 ga += __shfl_xor(ga, o, 64);
             }
-            const bool rot = act && al > floor2 && be > floor2 &&
-                             fabs(ga) > 1e-14 * sqrt(al * be);
+            const double nab = sqrt(al * be);
+            const bool rot = act && al > floor2 && be > floor2 && fabs(ga) > 1e-14 * nab;
             if (!__any(rot)) continue;
             any_rot = true;
             if (rot) {
+                worst = fmax(worst, fabs(ga) / nab);
                 // [p q] <- [p q] [c s; -s c] with p.q = 0 afterwards
                 const double zeta = (be - al) / (2.0 * ga);
                 double t;
@@ -344,7 +346,10 @@ __global__ __launch_bounds__(64) void gauges_onesided_kernel(
             }
             __syncthreads();  // the next step pairs columns written by other lanes
         }
-        if (!any_rot) break;
+        // Jacobi converges quadratically: once every rotated pair of a sweep
+        // was within 1e-10 of orthogonal, the next sweep would find them all
+        // below the 1e-14 threshold, so it is not run
+        if (!any_rot || wave_max(worst) < 1e-10) break;
     }
     // singular values: column norms; descending, ties lower index first
     for (int c = lane; c < m2; c += 64) {

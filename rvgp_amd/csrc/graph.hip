@@ -236,9 +236,17 @@ __global__ __launch_bounds__(kKnnMergeThreads) void knn_merge_kernel(
 
 // ---- symmetrisation ------------------------------------------------------
 
+// A neighbour id outside [0, n) (only possible for an nbr array that did not
+// come from a complete rvgp_knn, e.g. with a query order that is not a
+// permutation) is read as the row itself: it adds a duplicate self loop,
+// which the de-duplication removes, and no reverse edge -- every write stays
+// inside the row segments the counts reserved.
 __global__ void sym_count_kernel(const int* __restrict__ nbr, int n, int k, int* __restrict__ cnt) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e < n * k) atomicAdd(&cnt[nbr[e]], 1);
+    if (e < n * k) {
+        const int j = nbr[e];
+        if ((unsigned)j < (unsigned)n) atomicAdd(&cnt[j], 1);
+    }
     if (e < n) atomicAdd(&cnt[e], k + 1);
 }
 
@@ -277,7 +285,11 @@ __global__ void sym_fill_kernel(const int* __restrict__ nbr, int n, int k,
     int* dst = tmp + start[i] + pos;
     dst[0] = i;
     for (int q = 0; q < k; ++q) {
-        int j = nbr[(size_t)i * k + q];
+        const int j = nbr[(size_t)i * k + q];
+        if ((unsigned)j >= (unsigned)n) {
+            dst[1 + q] = i;
+            continue;
+        }
         dst[1 + q] = j;
         int pj = atomicAdd(&cursor[j], 1);
         tmp[start[j] + pj] = i;

@@ -82,8 +82,9 @@ def knn_device(xd, k, query_order=None):
     qp = None
     if query_order is not None:
         qp = torch.as_tensor(query_order, device=xd.device).to(torch.int32).contiguous()
-        if qp.shape != (n,):
-            raise ValueError("query_order must hold one entry per point")
+        if qp.shape != (n,) or not torch.equal(
+                torch.sort(qp).values, torch.arange(n, dtype=torch.int32, device=xd.device)):
+            raise ValueError("query_order must be a permutation of the points")
     _lib.call("rvgp_knn", _lib.ptr(xd), n, D, int(k), _lib.ptr(qp), _lib.ptr(nbr), _lib.ptr(ws),
               ws_bytes, _lib.stream())
     return nbr

@@ -142,6 +142,26 @@ def test_knn_rejects_non_finite(geo):
     x[7, 1] = 0.0
     with pytest.raises(ValueError):  # an order must cover every point
         geo.knn_device(torch.as_tensor(x, device="cuda"), 5, query_order=np.arange(49))
+    with pytest.raises(ValueError):  # ... exactly once
+        geo.knn_device(torch.as_tensor(x, device="cuda"), 5, query_order=np.r_[0, np.arange(49)])
+
+
+def test_symmetrize_out_of_range_ids_stay_in_bounds(geo):
+    """Neighbour ids outside [0, n) (an nbr array not produced by a complete
+    kNN) become self loops; the CSR equals the one without them."""
+    import torch
+    x = np.random.default_rng(2).normal(size=(300, 3))
+    nbr = cpu.knn(x, 6)
+    bad = nbr.copy()
+    bad[5, 2] = -7
+    bad[17, 0] = 10**6
+    rp, c = geo.symmetrize_device(torch.as_tensor(bad, device="cuda"))
+    keep = nbr.copy()
+    keep[5, 2] = 5
+    keep[17, 0] = 17
+    rpo, co = cpu.sym_csr(keep)
+    np.testing.assert_array_equal(rp.cpu().numpy(), rpo)
+    np.testing.assert_array_equal(c.cpu().numpy(), co)
 
 
 def test_manifold_graph_matches_networkx(geo):

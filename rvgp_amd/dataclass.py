@@ -229,6 +229,19 @@ class data:
             return torch.as_tensor(np.asarray(self._host["evals_Lc"], np.float64), device=self._X.device)
         return self._evals_Lc
 
+    def intrinsic_rows(self, nodes):
+        """Rows of the intrinsic eigenvectors U for the given nodes,
+        ((len(nodes)*d, k) f64), node-major -- or None once evecs_Lc was
+        assigned on the host.  Phi = blockdiag(T_i) U and T_i^T T_i = I, so
+        Phi^T Phi = U^T U and Phi^T y = U^T (T^T y): the weight-space
+        statistics need these d-row blocks instead of the D-row ones."""
+        if self._overridden("evecs_Lc"):
+            return None
+        nodes = torch.as_tensor(nodes, device=self._X.device).long()
+        d = self.dim_man
+        rows = (nodes.reshape(-1, 1) * d + torch.arange(d, device=nodes.device)).reshape(-1)
+        return self._U.index_select(0, rows).contiguous()
+
     def phi_rows(self, nodes, dtype=torch.float64):
         """Rows of evecs_Lc for the given nodes, (len(nodes)*D, k) on device."""
         D = self._X.shape[1]

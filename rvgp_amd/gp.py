@@ -63,11 +63,33 @@ class NLMLFunction(torch.autograd.Function):
         return g * grad, None, None
 
 
+def weight_space_stats(U, c, precision="fp64"):
+    """Phi^T Phi, Phi^T y and y^T y of a GP whose rows Phi = blockdiag(T_i) U
+    have orthonormal frames T_i, from the intrinsic rows U (m*d, k) and the
+    frame coordinates c = T^T y (m*d, R): Phi^T Phi = U^T U, Phi^T y = U^T c.
+    y^T y is not recoverable from c (the normal components of y count) and is
+    taken by the caller.  precision "mixed": FP32 U tiles with the products
+    accumulated in FP64 on the matrix cores, as for ambient rows."""
+    if precision == "fp64":
+        G = dense.gemm(U, U, transA=True, mode=dense.GEMM_F64)
+    else:
+        Uf = U.float()
+        G = dense.gemm(Uf, Uf, transA=True, mode=dense.GEMM_F32_ACC64, out_dtype=torch.float64)
+    b = dense.gemm(U, c, transA=True)
+    return G, b
+
+
 class GPREngine:
     """Hyperparameters theta = (nu, kappa, sigma_f, s2) = softplus(u) + lower."""
 
     def __init__(self, Phi, y, evals, n_rows, typ="matern", form="auto", precision="fp64",
-                 lower=(0.0, 0.0, 0.0, 1e-6), fs_max=8192, gram_mode=None):
+                 lower=(0.0, 0.0, 0.0, 1e-6), fs_max=8192, gram_mode=None, stats=None):
+        """Phi (M, k), y (M,) or (M, R); or Phi = y = None and stats = (G, b,
+        yy, M) -- the weight-space statistics Phi^T Phi (k, k), Phi^T y
+        (k, R), y^T y (1,) and the row count M -- for the weight form."""
+        if stats is not None:
+            self._init_from_stats(stats, evals, n_rows, typ, form, precision, lower)
+            return
         dev = Phi.device
         self.dev = dev
         self.M, self.k = Phi.shape
@@ -130,6 +152,44 @@ class GPREngine:
             self.Z = torch.empty_like(self.Phi)
             self.q = torch.empty(self.k, **f64)
             self.logdet = torch.empty(1, **f64)
+
+    def _init_from_stats(self, stats, evals, n_rows, typ, form, precision, lower):
+        G, b, yy, M = stats
+        if form not in ("auto", "weight"):
+            raise ValueError("weight-space statistics only serve the weight form")
+        dev = G.device
+        self.dev = dev
+        self.k = G.shape[0]
+        self.M = int(M)
+        self.typ = 0 if typ == "matern" else 1
+        self.n_rows = float(n_rows)
+        self.precision = precision
+        self.form = "weight"
+        self.gram_mode = "stats"
+        self.Phi = self.y = None
+        f64 = dict(dtype=torch.float64, device=dev)
+        self.G = G.to(torch.float64)
+        self.G = (0.5 * (self.G + self.G.T)).contiguous()
+        self.b = (b.reshape(-1, 1) if b.dim() == 1 else b).to(torch.float64).contiguous()
+        self.R = self.b.shape[1]
+        self.yy = yy.to(torch.float64).reshape(1).contiguous()
+        self.evals = evals.to(torch.float64).contiguous()
+        self.u = torch.zeros(4, **f64)
+        self.lower = torch.as_tensor(np.asarray(lower, np.float64), device=dev)
+        self.theta = torch.zeros(3, **f64)
+        self.s2 = torch.zeros(1, **f64)
+        self.S = torch.empty(self.k, **f64)
+        self.dS = torch.empty(3 * self.k, **f64)
+        self.gS = torch.empty(self.k, **f64)
+        self.out = torch.zeros(9, **f64)
+        self.status = torch.zeros(1, dtype=torch.int32, device=dev)
+        self._cs_ws = torch.empty(
+            _lib.lib().rvgp_colsumsq_ws_bytes(self.k, self.k) // 8 + 1, **f64)
+        self.A = torch.empty((self.k, self.k), **f64)
+        self.w = torch.empty((self.k, self.R), **f64)
+        self.Linv = torch.empty((self.k, self.k), **f64)
+        self.eye = torch.eye(self.k, **f64)
+        self.ainv = torch.empty(self.k, **f64)
 
     # ---------------------------------------------------------------- params
     def set_theta(self, theta):

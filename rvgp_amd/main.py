@@ -18,7 +18,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .gp import GPREngine
+from .gp import GPREngine, weight_space_stats
 from .kernels import RBF, ManifoldKernel, set_default_positive_minimum
 
 GAUSSIAN_VARIANCE_LOWER_BOUND = 1e-6  # gpflow.likelihoods.Gaussian default
@@ -130,25 +130,47 @@ def train_gp(data, train_ind=None, n_inducing_points=None, test_size=0.2, kernel
     dev = _lib.device()
     clock = _Clock(timings)
     nd_tr = torch.as_tensor(nodes_tr.astype(np.int32), device=dev)
-    Phi = data.phi_rows(nd_tr)
     V = data.vectors_device()
     if V is None:
         raise TypeError("data.vectors is None: nothing to fit")
     idx_tr = torch.as_tensor(nodes_tr, device=dev).long()
-    if V.dim() == 3:
-        # a batch (R, n, D) of fields observed at the same nodes: R
-        # right-hand sides of one GP with shared hyperparameters, NLML summed
-        y = V[:, idx_tr].permute(1, 2, 0).reshape(-1, V.shape[0]).contiguous()
+    D = V.shape[-1]
+    M = len(nodes_tr) * D
+    weight = n_inducing_points is None and (form == "weight" or (form == "auto" and M > 8192))
+    U = data.intrinsic_rows(idx_tr) if weight else None
+    if U is not None:
+        # weight form from the d-row intrinsic blocks (Phi^T Phi = U^T U):
+        # the D-row ambient rows are never formed
+        Phi = None
+        c = data._frame(0, V, data.dim_man)  # T_i^T v_i, (n, d) or (R, n, d)
+        if V.dim() == 3:
+            cy = c[:, idx_tr].permute(1, 2, 0).reshape(-1, V.shape[0]).contiguous()
+            yv = V[:, idx_tr]
+        else:
+            cy = c[idx_tr].reshape(-1, 1).contiguous()
+            yv = V[idx_tr]
+        yy = (yv * yv).sum().reshape(1)
     else:
-        y = V[idx_tr].reshape(-1)
+        Phi = data.phi_rows(nd_tr)
+        if V.dim() == 3:
+            # a batch (R, n, D) of fields observed at the same nodes: R
+            # right-hand sides of one GP with shared hyperparameters, NLML summed
+            y = V[:, idx_tr].permute(1, 2, 0).reshape(-1, V.shape[0]).contiguous()
+        else:
+            y = V[idx_tr].reshape(-1)
     noise0 = 1.0 if reference_semantics else float(noise_variance)
     lower = (kernel.nu.lower, kernel.kappa.lower, kernel.sigma_f.lower,
              GAUSSIAN_VARIANCE_LOWER_BOUND)
     if n_inducing_points is not None:
         return _train_sgpr(data, kernel, nodes_tr, nodes_te, Phi, y, V, int(n_inducing_points),
                            lower, noise0, epochs, optimizer, reference_semantics, verbose, clock)
-    eng = GPREngine(Phi, y, data.spectrum_device(), kernel.num_verticies, kernel.typ, form=form,
-                    precision=precision, lower=lower)
+    if Phi is None:
+        G, b = weight_space_stats(U, cy, precision)
+        eng = GPREngine(None, None, data.spectrum_device(), kernel.num_verticies, kernel.typ,
+                        form=form, precision=precision, lower=lower, stats=(G, b, yy, M))
+    else:
+        eng = GPREngine(Phi, y, data.spectrum_device(), kernel.num_verticies, kernel.typ, form=form,
+                        precision=precision, lower=lower)
     eng.set_theta([float(kernel.nu), float(kernel.kappa), float(kernel.sigma_f), noise0])
     clock.tick("gp_setup")
     if optimizer == "lbfgs":

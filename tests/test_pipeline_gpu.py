@@ -111,3 +111,64 @@ def test_mixed_precision_pipeline_within_1e4(pair):
     np.testing.assert_allclose(em, e64, rtol=0, atol=1e-5)
     assert np.abs(vm - v64).max() < 1e-4 * np.abs(v64).max()
     assert np.linalg.norm(mm - m64) / np.linalg.norm(m64) < 1e-4
+
+
+@pytest.mark.parametrize("fields", [1, 3])
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_weight_statistics_from_intrinsic_rows(pair, fields, prec):
+    """Phi^T Phi = U^T U and Phi^T y = U^T (T^T y) (orthonormal frames): the
+    engine built from the intrinsic statistics gives the NLML and gradient
+    of the one built from the ambient rows."""
+    import torch
+    from rvgp_amd.gp import GPREngine, weight_space_stats
+    d, _, _ = pair
+    rng = np.random.default_rng(8)
+    nodes = torch.as_tensor(rng.choice(800, size=500, replace=False), device="cuda")
+    V = torch.as_tensor(rng.normal(size=(fields, 800, 3)), device="cuda")
+    V = V[0] if fields == 1 else V
+    Phi = d.phi_rows(nodes.int())
+    if fields == 1:
+        y = V[nodes].reshape(-1)
+        c = d._frame(0, V, d.dim_man)[nodes].reshape(-1, 1).contiguous()
+        yv = V[nodes]
+    else:
+        y = V[:, nodes].permute(1, 2, 0).reshape(-1, fields).contiguous()
+        c = d._frame(0, V, d.dim_man)[:, nodes].permute(1, 2, 0).reshape(-1, fields).contiguous()
+        yv = V[:, nodes]
+    lam = d.spectrum_device()
+    a = GPREngine(Phi, y, lam, 800 * 3, form="weight", precision=prec)
+    G, b = weight_space_stats(d.intrinsic_rows(nodes), c, prec)
+    e = GPREngine(None, None, lam, 800 * 3, form="weight", precision=prec,
+                  stats=(G, b, (yv * yv).sum().reshape(1), 1500))
+    for th in [(1.5, 5.0, 1.0, 0.3), (2.2, 3.0, 0.7, 0.05)]:
+        a.set_theta(th)
+        e.set_theta(th)
+        oa, oe = a.evaluate().cpu().numpy(), e.evaluate().cpu().numpy()
+        tol = 1e-10 if prec == "fp64" else 1e-6
+        assert abs(oa[0] - oe[0]) < tol * abs(oa[0])
+        np.testing.assert_allclose(oe[1:], oa[1:], rtol=tol, atol=tol * np.abs(oa[1:]).max())
+
+
+def test_weight_form_fit_vs_oracle(pair):
+    """RVGP.fit through the intrinsic statistics (form='weight'), then the
+    ambient predictions vs the oracle at the fitted hyperparameters."""
+    import rvgp_amd as RVGP
+    d, o, X = pair
+    d.random_vector_field(seed=2)
+    o.random_vector_field(seed=2)
+    d.smooth_vector_field(t=10)
+    o.smooth_vector_field(t=10, method="expm")
+    train_ind = np.random.default_rng(6).choice(np.arange(800), size=400, replace=False)
+    gp = RVGP.fit(d, train_ind=train_ind, epochs=30, form="weight", verbose=False)
+    assert gp.engine.Phi is None and gp.engine.M == 320 * 3
+    test_ind = [int(i) for i in range(800) if i not in set(train_ind.tolist())]
+    mean, _ = gp.transform(d, test_ind)
+    th = gp.engine.theta_host()
+    tr, _ = cpu.train_test_split_indices(400, 0.2, 0)
+    nodes = train_ind[tr]
+    Phi = o.evecs_Lc.reshape(800, -1)[nodes].reshape(-1, 30)
+    y = o.vectors[nodes].reshape(-1, 1)
+    S, _ = cpu.spectral_filter(o.evals_Lc, th[0], th[1], th[2], 800 * 3)
+    Ps = o.evecs_Lc.reshape(800, -1)[test_ind].reshape(-1, 30)
+    mr, _ = cpu.gpr_predict(Phi, y, S, th[3], Ps)
+    assert np.linalg.norm(mean.ravel() - mr.ravel()) / np.linalg.norm(mr) < 1e-4

@@ -12,8 +12,10 @@ class CholeskyError(RuntimeError):
 
 
 def _ld(t):
-    assert t.dim() == 2 and t.stride(1) == 1, "row-major 2-D tensor with unit column stride"
-    return t.stride(0)
+    # a single column may carry any stride (torch treats it as contiguous)
+    assert t.dim() == 2 and (t.stride(1) == 1 or t.shape[1] == 1), \
+        "row-major 2-D tensor with unit column stride"
+    return max(t.stride(0), t.shape[1])
 
 
 def gemm(A, B, C=None, transA=False, transB=False, alpha=1.0, beta=0.0, kscale=None,

@@ -346,6 +346,27 @@ class GPREngine:
             var = d - v
         return (mean.reshape(-1) if self.R == 1 else mean), var
 
+    def predict_nodes(self, U_star, T_star):
+        """predict() at the ambient rows phi_ip = T_i[p] U_i of m nodes, from
+        their intrinsic rows U_star (m*d, k) and frames T_star (m, D, d):
+        mean_i = T_i (U_i w) and var_ip = s2 T_i[p] B_i T_i[p]^T with
+        B_i = (L_A^-1 U_i^T)^T (L_A^-1 U_i^T) (d x d), so the triangular solve
+        runs on m*d columns instead of m*D.  Weight-space system only."""
+        if not (self.form == "weight" or self.precision == "mixed"):
+            raise ValueError("predict_nodes needs the k x k weight-space system")
+        self.evaluate()
+        m, D, d = T_star.shape
+        T = T_star.to(torch.float64).contiguous()
+        mu = dense.gemm(U_star, self.w)                                  # (m*d, R)
+        mean = torch.bmm(T, mu.reshape(m, d, self.R))                    # (m, D, R)
+        Zt = U_star.T.contiguous()
+        dense.trsm(self.A, Zt)                                           # L_A^-1 U^T
+        Z = Zt.T.contiguous().reshape(m, d, self.k)
+        B = torch.bmm(Z, Z.transpose(1, 2).contiguous())                 # (m, d, d)
+        var = self.s2 * (torch.bmm(T, B) * T).sum(-1).reshape(-1)
+        mean = mean.reshape(m * D, self.R)
+        return (mean.reshape(-1) if self.R == 1 else mean), var
+
     # ---------------------------------------------------------------- optimisers
     def fit_lbfgs(self, maxiter=1000, disp=False):
         import scipy.optimize

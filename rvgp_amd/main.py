@@ -63,6 +63,22 @@ class manifold_GPR:
             return mean, var.reshape(-1, 1)
         return mean.reshape(-1, 1), var.reshape(-1, 1)
 
+    def predict_nodes(self, data, nodes):
+        """predict_f at the ambient rows of the given nodes: through the
+        intrinsic rows and frames when the weight-space system holds the fit,
+        else through data.phi_rows."""
+        nodes = torch.as_tensor(nodes, device=self.engine.dev).long()
+        e = self.engine
+        weight = hasattr(e, "predict_nodes") and (e.form == "weight" or e.precision == "mixed")
+        U = data.intrinsic_rows(nodes) if weight else None
+        if U is None:
+            return self.predict_f(data.phi_rows(nodes.int()))
+        T = data._gauges.index_select(0, nodes)[:, :, :data.dim_man]
+        mean, var = e.predict_nodes(U, T)
+        if e.R > 1:
+            return mean, var.reshape(-1, 1)
+        return mean.reshape(-1, 1), var.reshape(-1, 1)
+
     def transform(self, data, test_ind):
         """Predictive mean and variance at test nodes (main.py:102-116)."""
         first = test_ind[0]
@@ -72,13 +88,12 @@ class manifold_GPR:
             idx = np.asarray(test_ind)
             if idx.dtype == bool:  # a Python bool list indexes as a mask in numpy
                 idx = np.nonzero(idx)[0]
-            nodes = torch.as_tensor(idx.astype(np.int32), device=self.engine.dev)
-            test_x = data.phi_rows(nodes)
+            mean, var = self.predict_nodes(data, idx.astype(np.int64))
         elif isinstance(first, floats):
             test_x = np.asarray(test_ind, dtype=np.float64).reshape(-1, self.engine.k)
+            mean, var = self.predict_f(test_x)
         else:  # np.int64 / np.bool_ match neither branch of main.py:104-109
             raise UnboundLocalError("local variable 'test_x' referenced before assignment")
-        mean, var = self.predict_f(test_x)
         if self.engine.R > 1:
             # a batch of R fields fitted with shared hyperparameters:
             # mean (R, len(test_ind), D), variance (len(test_ind), D)
@@ -195,8 +210,8 @@ def train_gp(data, train_ind=None, n_inducing_points=None, test_size=0.2, kernel
     model = manifold_GPR(data, kernel, nodes_tr, eng, reference_semantics)
 
     if len(nodes_te):
-        mean, _ = model.predict_f(data.phi_rows(torch.as_tensor(nodes_te.astype(np.int32), device=dev)))
         idx_te = torch.as_tensor(nodes_te, device=dev).long()
+        mean, _ = model.predict_nodes(data, idx_te)
         out_test = (V[:, idx_te].permute(1, 2, 0).reshape(-1, V.shape[0]) if V.dim() == 3
                     else V[idx_te].reshape(-1, 1))
         # np.linalg.norm(out_test - out_pred, axis=1).mean() on (rows, 1) arrays

@@ -172,3 +172,28 @@ def test_weight_form_fit_vs_oracle(pair):
     Ps = o.evecs_Lc.reshape(800, -1)[test_ind].reshape(-1, 30)
     mr, _ = cpu.gpr_predict(Phi, y, S, th[3], Ps)
     assert np.linalg.norm(mean.ravel() - mr.ravel()) / np.linalg.norm(mr) < 1e-4
+
+
+@pytest.mark.parametrize("fields", [1, 2])
+def test_predict_nodes_equals_ambient_rows(pair, fields):
+    """Mean and variance at test nodes from intrinsic rows and frames equal
+    predict() on the nodes' ambient rows Phi = T U."""
+    import torch
+    from rvgp_amd.gp import GPREngine, weight_space_stats
+    d, _, _ = pair
+    rng = np.random.default_rng(12)
+    nodes = torch.as_tensor(rng.choice(800, size=300, replace=False), device="cuda")
+    V = torch.as_tensor(rng.normal(size=(fields, 800, 3)), device="cuda")
+    c = d._frame(0, V, d.dim_man)[:, nodes].permute(1, 2, 0).reshape(-1, fields).contiguous()
+    yv = V[:, nodes]
+    G, b = weight_space_stats(d.intrinsic_rows(nodes), c, "fp64")
+    e = GPREngine(None, None, d.spectrum_device(), 800 * 3, form="weight",
+                  stats=(G, b, (yv * yv).sum().reshape(1), 900))
+    e.set_theta((1.7, 4.0, 0.9, 0.2))
+    test = torch.as_tensor(np.setdiff1d(np.arange(800), nodes.cpu().numpy())[:200], device="cuda")
+    m1, v1 = e.predict(d.phi_rows(test.int()))
+    T = d._gauges.index_select(0, test)[:, :, :d.dim_man]
+    m2, v2 = e.predict_nodes(d.intrinsic_rows(test), T)
+    m1, m2 = m1.cpu().numpy(), m2.cpu().numpy()
+    np.testing.assert_allclose(m2, m1, rtol=1e-10, atol=1e-12 * np.abs(m1).max())
+    np.testing.assert_allclose(v2.cpu().numpy(), v1.cpu().numpy(), rtol=1e-9, atol=1e-14)

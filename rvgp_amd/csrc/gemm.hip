@@ -637,6 +637,21 @@ static int gemm_launch(int transA, int transB, int M, int N, int K, double alpha
 // a Rayleigh-Ritz step of an FP32 block.
 constexpr int kF32Chunk = 4096;
 
+// Workgroups of the 128 x 128 FP32 kernel resident at once on the device.
+static int f32_big_slots() {
+    static int slots = 0;  // the same for every device of one process's type
+    if (slots == 0) {
+        int dev = 0, per = 0;
+        hipDeviceProp_t prop;
+        if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, gemm_big_kernel<Mode::F32>, GT, 0) !=
+                hipSuccess)
+            return 768;
+        slots = (per > 0 ? per : 1) * prop.multiProcessorCount;
+    }
+    return slots;
+}
+
 static GemmPlan gram_f32_plan(int M, int K, int split_k) {
     GemmPlan p = gemm_plan(RVGP_GEMM_F32, M, M, K, Tr<Mode::F32>::BK, split_k);
     const int need = (K + kF32Chunk - 1) / kF32Chunk;
@@ -644,6 +659,16 @@ static GemmPlan gram_f32_plan(int M, int K, int split_k) {
         p.small = false;
         p.big = M >= 96;
         p.split = need < 1 ? 1 : need;
+        if (p.big && split_k <= 0) {
+            // Fill whole rounds of resident workgroups: with T upper tiles and
+            // `slots` resident at once, S chunks take ceil(T S / slots) rounds
+            // of K / S rows each, so S is raised to the largest count that
+            // still fits the rounds the minimum count needs.
+            const long nt = cdiv(M, BBM), T = nt * (nt + 1) / 2, slots = f32_big_slots();
+            const long rounds = (T * p.split + slots - 1) / slots;
+            const long s = slots * rounds / T;
+            if (s > p.split && s <= 4 * (long)p.split) p.split = (int)s;
+        }
         int kchunk = (K + p.split - 1) / p.split;
         kchunk = ((kchunk + Tr<Mode::F32>::BK - 1) / Tr<Mode::F32>::BK) * Tr<Mode::F32>::BK;
         p.kchunk = kchunk < Tr<Mode::F32>::BK ? Tr<Mode::F32>::BK : kchunk;

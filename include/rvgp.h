@@ -180,8 +180,10 @@ int rvgp_diffusion_normalise(size_t rows, int d, double* out, int ldo, const dou
 
 /* a9  Ambient eigenvector rows Phi = T_i U_i for the listed nodes
  * (dataclass.py:57-59): out ((m*D), k) rows r*D+p, f32 if out_f32 else f64;
- * U (n*d, k) f64 with row stride ldu. */
-int rvgp_phi_rows(int out_f32, int m, int D, int d, const double* T, int ldt, int k,
+ * U (n*d, k) f64 with row stride ldu.  nodes[r] must lie in [0, n): a node
+ * outside it reads no memory and its D rows are NaN (the host wrappers raise
+ * IndexError before launching, as numpy indexing does). */
+int rvgp_phi_rows(int out_f32, int m, int n, int D, int d, const double* T, int ldt, int k,
                   const double* U, int ldu, const int* nodes, void* out, int ldo, void* stream);
 
 /* ---------------------------------------------------------------------

@@ -23,8 +23,20 @@ tests/test_oracle.py:
                       on the golden R; ring-holonomy closed form.
   compute_spectrum    geometry.py:66-80     pinned: ``numpy.linalg.eigh`` and the
                       ring-graph closed form 2-2cos(2 pi j/n).
-  smooth_vector_field dataclass.py:105-120, smoothing.py:8-64  pinned:
-                      ``scipy.linalg.expm`` vs eigen form; exp(-tL)1 = 1.
+  heat                smoothing.py:8-15     pinned: ``scipy.linalg.expm`` vs the
+                      eigen form; exp(-tL)1 = 1.
+  vector_diffusion    smoothing.py:37-64    pinned: closed form on a trivial
+                      connection (L (x) I, one-direction field: the result is
+                      e^{-tL}a along v); for any connection |out_i| =
+                      (e^{-tL}|x|)_i / (e^{-tL}1)_i with out_i parallel to
+                      (e^{-tLc}x)_i from Lc's own eigendecomposition; t -> 0.
+  project_to_manifold geometry.py:165-168   pinned: projector identities
+                      (P^2 = P, P T = T, residual orthogonal to T, |Px| <= |x|).
+  express_in_local_frame geometry.py:171-175  pinned: T^T (T c) = c,
+                      T (T^T x) = P x, c -> O^T c under the gauge T -> T O.
+  Data (Phi assembly) dataclass.py:57-59    pinned: Phi^T Phi = U^T U, rows in
+                      the tangent planes, B Lc B^T Phi = Phi diag(evals_Lc).
+  smooth_vector_field dataclass.py:105-120  composition of the three above.
   spectral_filter     kernels.py:41-53      pinned: closed form + finite diffs.
   gpr_*               GPflow GPR (main.py:98-116)  pinned: textbook GPR
                       (Rasmussen & Williams Alg. 2.1) vs a dense
@@ -32,6 +44,10 @@ tests/test_oracle.py:
                       finite differences; GPflow itself is absent, so the
                       GPflow-specific optimiser trajectory is parity unpinned.
   train_test_split    main.py:40-45         pinned: sklearn ``train_test_split``.
+  fit_lbfgs           main.py:87-95         pinned: the returned point is a
+                      stationary minimum (gradient ~0 on the free parameters,
+                      no +-1e-3 coordinate step lowers the NLML); the GPflow
+                      trajectory itself is parity unpinned.
 """
 import ctypes
 import os

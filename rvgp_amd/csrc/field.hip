@@ -65,7 +65,7 @@ __global__ void diffusion_normalise_kernel(size_t rows, int d, double* __restric
 
 // Phi rows: out[(r*D + p), j] = sum_q T[node_r][p][q] * U[node_r*d + q][j]
 template <typename TO>
-__global__ void phi_rows_kernel(int m, int D, int d, int ldt, int k,
+__global__ void phi_rows_kernel(int m, int n, int D, int d, int ldt, int k,
                                 const double* __restrict__ T, const double* __restrict__ U,
                                 int ldu, const int* __restrict__ nodes, TO* __restrict__ out,
                                 int ldo) {
@@ -73,6 +73,11 @@ __global__ void phi_rows_kernel(int m, int D, int d, int ldt, int k,
     if (j >= k) return;
     for (int r = blockIdx.y; r < m; r += gridDim.y) {
         const int i = nodes[r];
+        if ((unsigned)i >= (unsigned)n) {
+            // a node outside [0, n): the row reads nothing and is NaN
+            for (int p = 0; p < D; ++p) out[(size_t)(r * D + p) * ldo + j] = (TO)__builtin_nan("");
+            continue;
+        }
         const double* Ti = T + (size_t)i * D * ldt;
         double u[8];
         for (int q = 0; q < d; ++q) u[q] = U[(size_t)(i * d + q) * ldu + j];
@@ -110,17 +115,18 @@ extern "C" int rvgp_diffusion_normalise(size_t rows, int d, double* out, int ldo
     return RVGP_OK;
 }
 
-extern "C" int rvgp_phi_rows(int out_f32, int m, int D, int d, const double* T, int ldt, int k,
-                             const double* U, int ldu, const int* nodes, void* out, int ldo,
+extern "C" int rvgp_phi_rows(int out_f32, int m, int n, int D, int d, const double* T, int ldt,
+                             int k, const double* U, int ldu, const int* nodes, void* out, int ldo,
                              void* stream) {
-    RVGP_CHECK_ARG(d >= 1 && d <= 8 && ldt >= d && ldu >= k && ldo >= k, "rvgp_phi_rows: bad sizes");
+    RVGP_CHECK_ARG(d >= 1 && d <= 8 && ldt >= d && ldu >= k && ldo >= k && n >= 0,
+                   "rvgp_phi_rows: bad sizes");
     if (m == 0 || k == 0) return RVGP_OK;
     dim3 g(cdiv(k, 256), m < 65535 ? m : 65535);
     if (out_f32)
-        hipLaunchKernelGGL(phi_rows_kernel<float>, g, dim3(256), 0, as_stream(stream), m, D, d, ldt, k,
+        hipLaunchKernelGGL(phi_rows_kernel<float>, g, dim3(256), 0, as_stream(stream), m, n, D, d, ldt, k,
                            T, U, ldu, nodes, static_cast<float*>(out), ldo);
     else
-        hipLaunchKernelGGL(phi_rows_kernel<double>, g, dim3(256), 0, as_stream(stream), m, D, d, ldt,
+        hipLaunchKernelGGL(phi_rows_kernel<double>, g, dim3(256), 0, as_stream(stream), m, n, D, d, ldt,
                            k, T, U, ldu, nodes, static_cast<double*>(out), ldo);
     RVGP_LAUNCH_CHECK();
     return RVGP_OK;

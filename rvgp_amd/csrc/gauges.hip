@@ -292,7 +292,6 @@ __global__ __launch_bounds__(64) void gauges_onesided_kernel(
     const int r0 = j * RPL;
     for (int sweep = 0; sweep < 40; ++sweep) {
         bool any_rot = false;
-        double worst = 0.0;  // largest |cos| between a rotated pair in this sweep
         for (int step = 0; step < m2 - 1; ++step) {
             int p = 0, q = 1;
             if (act) {
@@ -329,7 +328,6 @@ __global__ __launch_bounds__(64) void gauges_onesided_kernel(
             if (!__any(rot)) continue;
             any_rot = true;
             if (rot) {
-                worst = fmax(worst, fabs(ga) / nab);
                 // [p q] <- [p q] [c s; -s c] with p.q = 0 afterwards
                 const double zeta = (be - al) / (2.0 * ga);
                 double t;
@@ -346,10 +344,15 @@ __global__ __launch_bounds__(64) void gauges_onesided_kernel(
             }
             __syncthreads();  // the next step pairs columns written by other lanes
         }
-        // Jacobi converges quadratically: once every rotated pair of a sweep
-        // was within 1e-10 of orthogonal, the next sweep would find them all
-        // below the 1e-14 threshold, so it is not run
-        if (!any_rot || wave_max(worst) < 1e-10) break;
+        // Stop only after a sweep that rotated nothing: every pair was then
+        // within 1e-14 of orthogonal when visited and nothing moved after.
+        // A sweep whose rotated pairs were all nearly orthogonal is not
+        // enough: between columns of nearly equal norm a small cosine still
+        // means a large rotation angle, which mixes a pair orthogonalised
+        // earlier in the sweep with one not yet visited, leaving residual
+        // cosines of the size of the last sweep's (not its square), and the
+        // weight-space statistics rely on T^T T = I.
+        if (!any_rot) break;
     }
     // singular values: column norms; descending, ties lower index first
     for (int c = lane; c < m2; c += 64) {

@@ -638,6 +638,8 @@ static int gemm_launch(int transA, int transB, int M, int N, int K, double alpha
 constexpr int kF32Chunk = 4096;
 
 // Workgroups of the 128 x 128 FP32 kernel resident at once on the device.
+// Queried once and cached, the fallback included, so that the workspace size
+// query and the launch that follows it always plan the same split.
 static int f32_big_slots() {
     static int slots = 0;  // the same for every device of one process's type
     if (slots == 0) {
@@ -646,8 +648,9 @@ static int f32_big_slots() {
         if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess ||
             hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, gemm_big_kernel<Mode::F32>, GT, 0) !=
                 hipSuccess)
-            return 768;
-        slots = (per > 0 ? per : 1) * prop.multiProcessorCount;
+            slots = 768;
+        else
+            slots = (per > 0 ? per : 1) * prop.multiProcessorCount;
     }
     return slots;
 }

@@ -144,6 +144,7 @@ class data:
         self._graph = G
         self._gauges = gauges
         self._tangents_full = tangents
+        self._Sigma = Sigma
         self._R = conn["R"]
         self._Lb, self._Lcb = Lb, Lcb
         self._evals_L = wL
@@ -238,6 +239,8 @@ class data:
         if self._overridden("evecs_Lc"):
             return None
         nodes = torch.as_tensor(nodes, device=self._X.device).long()
+        if nodes.numel() and (int(nodes.min()) < 0 or int(nodes.max()) >= self.n):
+            raise IndexError(f"node index out of bounds for axis 0 with size {self.n}")
         d = self.dim_man
         rows = (nodes.reshape(-1, 1) * d + torch.arange(d, device=nodes.device)).reshape(-1)
         return self._U.index_select(0, rows).contiguous()
@@ -255,7 +258,7 @@ class data:
         k = self._U.shape[1]
         out = torch.empty((m * D, k), dtype=dtype, device=self._X.device)
         T = self._gauges
-        _lib.call("rvgp_phi_rows", int(dtype == torch.float32), m, D, self.dim_man, _lib.ptr(T),
+        _lib.call("rvgp_phi_rows", int(dtype == torch.float32), m, self.n, D, self.dim_man, _lib.ptr(T),
                   T.shape[2], k, _lib.ptr(self._U), self._U.stride(0), _lib.ptr(nodes),
                   _lib.ptr(out), k, _lib.stream())
         return out

@@ -12,9 +12,15 @@ class CholeskyError(RuntimeError):
 
 
 def _ld(t):
-    # a single column may carry any stride (torch treats it as contiguous)
+    """Row stride of a row-major 2-D operand.  A single column may carry any
+    column stride (torch treats it as contiguous); a row stride below the row
+    width (an expanded / broadcast view, stride 0) would make the kernels read
+    rows that are not stored, and is refused -- call .contiguous() first."""
     assert t.dim() == 2 and (t.stride(1) == 1 or t.shape[1] == 1), \
         "row-major 2-D tensor with unit column stride"
+    if t.shape[0] > 1 and t.stride(0) < t.shape[1]:
+        raise ValueError(f"operand rows overlap (shape {tuple(t.shape)}, row stride "
+                         f"{t.stride(0)}): pass a .contiguous() tensor")
     return max(t.stride(0), t.shape[1])
 
 

@@ -24,6 +24,26 @@ from .kernels import RBF, ManifoldKernel, set_default_positive_minimum
 GAUSSIAN_VARIANCE_LOWER_BOUND = 1e-6  # gpflow.likelihoods.Gaussian default
 
 
+def checked_nodes(idx, n):
+    """Node indices as numpy indexing reads them (main.py:27, 41, 104):
+    integer indices in [-n, n), negatives wrapped to idx + n; anything else
+    raises IndexError before a kernel sees it (an index outside [0, n) would
+    otherwise be an out-of-bounds device read)."""
+    idx = np.asarray(idx)
+    if idx.dtype == bool:
+        if idx.shape != (n,):
+            raise IndexError(f"boolean index did not match indexed array along axis 0; size of "
+                             f"axis is {n} but size of corresponding boolean axis is {idx.size}")
+        return np.nonzero(idx)[0].astype(np.int64)
+    if idx.size and not np.issubdtype(idx.dtype, np.integer):
+        raise IndexError("arrays used as indices must be of integer (or boolean) type")
+    idx = idx.astype(np.int64).reshape(-1)
+    bad = (idx < -n) | (idx >= n)
+    if bad.any():
+        raise IndexError(f"index {int(idx[bad][0])} is out of bounds for axis 0 with size {n}")
+    return np.where(idx < 0, idx + n, idx)
+
+
 def train_test_split_indices(n, test_size=0.2, seed=0):
     """Index rule of sklearn.model_selection.train_test_split(random_state=seed)
     for a float test_size: n_test = ceil(test_size * n), permutation from
@@ -67,7 +87,9 @@ class manifold_GPR:
         """predict_f at the ambient rows of the given nodes: through the
         intrinsic rows and frames when the weight-space system holds the fit,
         else through data.phi_rows."""
-        nodes = torch.as_tensor(nodes, device=self.engine.dev).long()
+        if isinstance(nodes, torch.Tensor):
+            nodes = nodes.cpu().numpy()
+        nodes = torch.as_tensor(checked_nodes(nodes, data.n), device=self.engine.dev)
         e = self.engine
         weight = hasattr(e, "predict_nodes") and (e.form == "weight" or e.precision == "mixed")
         U = data.intrinsic_rows(nodes) if weight else None
@@ -85,10 +107,8 @@ class manifold_GPR:
         ints = (int,) if self.reference_semantics else (int, np.integer)
         floats = (float,) if self.reference_semantics else (float, np.floating)
         if isinstance(first, ints):
-            idx = np.asarray(test_ind)
-            if idx.dtype == bool:  # a Python bool list indexes as a mask in numpy
-                idx = np.nonzero(idx)[0]
-            mean, var = self.predict_nodes(data, idx.astype(np.int64))
+            # a Python bool list indexes as a mask in numpy
+            mean, var = self.predict_nodes(data, checked_nodes(test_ind, data.n))
         elif isinstance(first, floats):
             test_x = np.asarray(test_ind, dtype=np.float64).reshape(-1, self.engine.k)
             mean, var = self.predict_f(test_x)
@@ -113,9 +133,7 @@ def train_gp(data, train_ind=None, n_inducing_points=None, test_size=0.2, kernel
     """Fit the vector-field GP (main.py:11-84)."""
     if train_ind is None:
         train_ind = np.arange(data.n)
-    train_ind = np.asarray(train_ind)
-    if train_ind.dtype == bool:
-        train_ind = np.nonzero(train_ind)[0]
+    train_ind = checked_nodes(train_ind, data.n)
     if kernel is None:
         kernel = ManifoldKernel(data, nu=3 / 2, kappa=5, typ='matern', sigma_f=1.)
     elif isinstance(kernel, str) and kernel == 'rbf':
@@ -308,7 +326,7 @@ class rbf_GPR:
         ints = (int,) if self.reference_semantics else (int, np.integer)
         floats = (float,) if self.reference_semantics else (float, np.floating)
         if isinstance(first, ints):
-            idx = torch.as_tensor(np.asarray(test_ind).astype(np.int64), device=self.engine.dev)
+            idx = torch.as_tensor(checked_nodes(test_ind, data.n), device=self.engine.dev)
             if self.reference_semantics:
                 E = torch.as_tensor(np.asarray(data.evecs_Lc), device=self.engine.dev)
                 test_x = E.reshape(data.n, -1)[idx].reshape(-1, E.shape[1])

@@ -124,3 +124,22 @@ def test_gemm_sym_matches_full_product(K, M):
             iu = np.triu_indices(M)
             np.testing.assert_allclose(C[iu], ref[iu], rtol=0, atol=tol * np.abs(ref).max())
             assert np.array_equal(C, C.T)
+
+
+def test_gemm_sym_f32_raised_split_matches_full_product():
+    """The FP32 Gram's K split is raised past ceil(K/4096) chunks to fill
+    whole rounds of resident workgroups (gram_f32_plan): at K = 65536,
+    M = 576 the workspace query reports the raised split, and the product
+    computed with it equals A^T B to the FP32-chunk tolerance."""
+    import torch
+    from rvgp_amd import _lib, dense
+    K, M = 65536, 576
+    ws = _lib.lib().rvgp_gemm_ws_bytes(dense.GEMM_F32, M, M, K, 0)
+    assert ws > -(-K // 4096) * M * M * 4, ws
+    rng = np.random.default_rng(3)
+    Y = rng.normal(size=(K, M)).astype(np.float32)
+    C = dense.gemm_sym(_t(Y, torch.float32), _t(Y, torch.float32)).cpu().numpy()
+    Yd = Y.astype(np.float64)
+    ref = Yd.T @ Yd
+    np.testing.assert_allclose(C, ref, rtol=0, atol=1e-5 * np.abs(ref).max())
+    assert np.array_equal(C, C.T)

@@ -11,7 +11,7 @@ import scipy.linalg
 import scipy.sparse as sp
 
 from oracle import cpu
-from tests.util import sphere, torus
+from tests.util import flat_torus3, sphere, torus
 
 pytestmark = pytest.mark.gpu
 
@@ -111,6 +111,31 @@ def test_sphere_connection_laplacian_vs_eigh(precision):
     assert abs(w1[0]) < 1e-9 * ub
     np.testing.assert_allclose(w1[1:], wr1[1:k], rtol=1e-6)
     assert cluster_angles(wr1, Vr1, V1.cpu().numpy(), k) < 1e-4
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_flat_3torus_connection_laplacian_b3_vs_eigh(precision):
+    """The headline operator's kind (3-torus in R^32, k = 20, K = 30 < D,
+    dim_man = 3: 3 x 3 blocks) at n = 700: eigenvalues and cluster subspaces
+    of the b = 3 block solver vs numpy.linalg.eigh of the dense FP64 matrix."""
+    from rvgp_amd.eigsolve import chfsi
+    X = flat_torus3(700, 2)
+    rp, c = cpu.sym_csr(cpu.knn(X, 20))
+    T, S = cpu.tangent_frames(X, rp, c, 32, 30, allow_rank_deficient_k=True)
+    assert cpu.manifold_dimension(S, 0.8) == 3
+    R = cpu.connections(np.ascontiguousarray(T[:, :, :3]), rp, c)
+    A = _bsr_from_blocks(rp, c, cpu.lc_blocks(rp, c, R), 3)
+    Lc = cpu.connection_laplacian(rp, c, R)
+    w_ref, V_ref = np.linalg.eigh(Lc.toarray())
+    # a cut at a spectral gap (the first k eigenvectors then span one space)
+    gaps = np.diff(w_ref[50:70])
+    k = 51 + int(np.argmax(gaps))
+    w, V, st = chfsi(A, k, precision=precision)
+    w, V = w.cpu().numpy(), V.cpu().numpy()
+    ub = A.gershgorin_bound()
+    err = np.abs(w - w_ref[:k]) / np.maximum(np.abs(w_ref[:k]), 1e-9 * ub)
+    assert err.max() < 1e-6, (err.max(), st)
+    assert cluster_angles(w_ref, V_ref, V, k) < 1e-4
 
 
 def test_eig_rejects_k_ge_n_minus_1():

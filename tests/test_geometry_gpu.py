@@ -255,13 +255,16 @@ def test_gauges_rank_deficient_raises():
         ptu_dijkstra.tangent_frames(X, G, 3, 15)
 
 
-def test_gauges_wide_ambient_gram_path():
-    """D > K+1 (the 3-torus in R^32 shape): Gram-side eigensolve."""
+@pytest.mark.parametrize("k,K", [(10, 15), (20, 30)])
+def test_gauges_wide_ambient_gram_path(k, K):
+    """D > K+1 (the 3-torus in R^32 shape; k = 20 gives the headline's
+    K = 30): one-sided Jacobi on the neighbourhood columns.  Singular values
+    and tangent projectors vs the oracle's LAPACK SVD, and the frames
+    orthonormal to rounding (the weight-space statistics use T^T T = I)."""
     import torch
     from rvgp_amd import ptu_dijkstra
     from rvgp_amd.graph import SymGraph
     X = flat_torus3(600, 5, D=32)
-    k, K = 10, 15
     rp, c = cpu.sym_csr(cpu.knn(X, k))
     g = SymGraph(torch.as_tensor(rp, device="cuda"), torch.as_tensor(c, device="cuda"), len(X))
     T, S = ptu_dijkstra.tangent_frames_device(torch.as_tensor(X, device="cuda"), g, K, K + 1, K + 1, K)
@@ -272,6 +275,9 @@ def test_gauges_wide_ambient_gram_path():
     P = np.einsum("nij,nkj->nik", T[:, :, :d], T[:, :, :d])
     Pr = np.einsum("nij,nkj->nik", Tr[:, :, :d], Tr[:, :, :d])
     assert np.abs(P - Pr).max() < 1e-8
+    # frames of the K singular directions with non-zero singular values
+    TT = np.einsum("npi,npj->nij", T[:, :, :K], T[:, :, :K])
+    assert np.abs(TT - np.eye(K)).max() < 1e-12
 
 
 # ---------------------------------------------------------------- a5-a7 connections / Laplacians

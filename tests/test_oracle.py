@@ -351,3 +351,156 @@ def test_sgpr_oracle_bound_properties():
     m2, v2 = cpu.gpr_predict(Phi, y, S, 0.3, Ps)
     np.testing.assert_allclose(m1.ravel(), m2.ravel(), rtol=1e-7, atol=1e-9)
     np.testing.assert_allclose(v1.ravel(), v2.ravel(), rtol=1e-6, atol=1e-9)
+
+
+# ---------------------------------------------------------------- a9-a11 frames, Phi, diffusion
+def _orthonormal_frames(n, D, d, seed):
+    rng = np.random.default_rng(seed)
+    T = np.empty((n, D, d))
+    for i in range(n):
+        q, _ = np.linalg.qr(rng.normal(size=(D, d)))
+        T[i] = q
+    return T
+
+
+def test_frame_maps_pinned_by_projector_identities():
+    """project_to_manifold (geometry.py:165-168) is the orthogonal projector
+    T T^T: idempotent, identity on the tangent plane, residual orthogonal to
+    it; express_in_local_frame (geometry.py:171-175) and its reverse are
+    T^T and T, so T^T (T c) = c and T (T^T x) = P x."""
+    n, D, d = 50, 5, 2
+    T = _orthonormal_frames(n, D, d, 0)
+    rng = np.random.default_rng(1)
+    x = rng.normal(size=(n, D))
+    c = rng.normal(size=(n, d))
+    Px = cpu.project_to_manifold(x, T)
+    np.testing.assert_allclose(cpu.project_to_manifold(Px, T), Px, atol=1e-14)
+    tang = np.einsum("nij,nj->ni", T, c)  # tangent vectors built without the oracle
+    np.testing.assert_allclose(cpu.project_to_manifold(tang, T), tang, atol=1e-14)
+    res = x - Px
+    assert np.abs(np.einsum("nij,ni->nj", T, res)).max() < 1e-14
+    assert np.all(np.linalg.norm(Px, axis=1) <= np.linalg.norm(x, axis=1) + 1e-14)
+    loc = cpu.express_in_local_frame(tang, T)
+    np.testing.assert_allclose(loc, c, atol=1e-14)
+    back = cpu.express_in_local_frame(cpu.express_in_local_frame(x, T), T, reverse=True)
+    np.testing.assert_allclose(back, Px, atol=1e-14)
+    # local coordinates rotate with the gauge: T -> T O gives c -> O^T c
+    O = _orthonormal_frames(n, d, d, 2)
+    TO = np.einsum("nij,njk->nik", T, O)
+    np.testing.assert_allclose(cpu.express_in_local_frame(x, TO),
+                               np.einsum("nji,nj->ni", O, cpu.express_in_local_frame(x, T)),
+                               atol=1e-14)
+
+
+def test_data_phi_is_the_ambient_eigenbasis():
+    """Data's Phi = einsum(gauges, U) (dataclass.py:57-59): Phi^T Phi = U^T U
+    (orthonormal frames), every node's rows lie in its tangent plane, and
+    Phi diagonalises the ambient connection Laplacian B Lc B^T
+    (B = blockdiag(T_i)): B Lc B^T Phi = Phi diag(evals_Lc).  A transposed
+    frame or a contraction over the wrong index fails all three."""
+    X = sphere(300, 3)
+    o = cpu.Data(X, n_neighbors=10, n_eigenpairs=12, dense_eigh=True)
+    n, D, d, k = o.n, X.shape[1], o.dim_man, o.U.shape[1]
+    Phi, U = o.evecs_Lc, o.U
+    np.testing.assert_allclose(Phi.T @ Phi, U.T @ U, atol=1e-10 * n)
+    P = np.einsum("nij,nkj->nik", o.gauges, o.gauges)
+    Pr = Phi.reshape(n, D, k)
+    np.testing.assert_allclose(np.einsum("nij,njk->nik", P, Pr), Pr, atol=1e-12)
+    B = sp.block_diag([o.gauges[i] for i in range(n)], format="csr")
+    Aamb = B @ o.Lc @ B.T
+    np.testing.assert_allclose(Aamb @ Phi, Phi * o.evals_Lc, atol=1e-9 * np.abs(Phi).max())
+
+
+def _trivial_connection(rp, cols, d):
+    R = np.zeros((len(cols), d, d))
+    R[:] = np.eye(d)
+    return R
+
+
+def test_vector_diffusion_pinned_by_closed_forms():
+    """vector_diffusion (smoothing.py:37-64, normalise=True):
+    * trivial connection (R = I, Lc = L (x) I) and a field a_i v of one
+      direction with positive magnitudes a: the result is (e^{-tL} a)_i v,
+      with e^{-tL} from an independent eigendecomposition of L -- a
+      normalisation by |c| instead of |out|, or by e^{-tL}|c| without
+      e^{-tL}1, fails it;
+    * any connection: |result_i| = (e^{-tL}|x|)_i / (e^{-tL} 1)_i and
+      result_i is parallel to (e^{-tLc} x)_i (Lc's heat kernel from its own
+      eigendecomposition);
+    * t -> 0 returns x."""
+    X = torus(200, 4)
+    rp, cols = cpu.sym_csr(cpu.knn(X, 8))
+    n, d = len(rp) - 1, 2
+    L = cpu.laplacian(rp, cols).toarray()
+    w, V = np.linalg.eigh(L)
+    def heatL(t, y):
+        return (V @ (np.exp(-t * w)[:, None] * (V.T @ y.reshape(n, -1)))).reshape(y.shape)
+
+    Lc_triv = cpu.connection_laplacian(rp, cols, _trivial_connection(rp, cols, d))
+    np.testing.assert_allclose(Lc_triv.toarray(), np.kron(L, np.eye(d)), atol=1e-14)
+    rng = np.random.default_rng(5)
+    a = rng.uniform(0.2, 3.0, n)
+    v = np.array([0.6, -0.8])
+    x = a[:, None] * v
+    for t in (0.3, 2.0):
+        out = cpu.vector_diffusion(x, t, Lc_triv, cpu.laplacian(rp, cols), method="eigh")
+        np.testing.assert_allclose(out, heatL(t, a)[:, None] * v, atol=1e-12)
+    # a random orthogonal connection (R_ji = R_ij^T)
+    Rb = np.empty((len(cols), d, d))
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    ang = {}
+    for e, (i, j) in enumerate(zip(rows, cols)):
+        key = (min(i, j), max(i, j))
+        if key not in ang:
+            ang[key] = 0.0 if i == j else rng.uniform(-np.pi, np.pi)
+        th = ang[key] if i <= j else -ang[key]
+        Rb[e] = [[np.cos(th), -np.sin(th)], [np.sin(th), np.cos(th)]]
+    Lc = cpu.connection_laplacian(rp, cols, Rb)
+    wc, Vc = np.linalg.eigh(Lc.toarray())
+    x = rng.normal(size=(n, d))
+    t = 0.7
+    out = cpu.vector_diffusion(x, t, Lc, cpu.laplacian(rp, cols), method="expm")
+    hc = (Vc @ (np.exp(-t * wc)[:, None] * (Vc.T @ x.reshape(-1, 1)))).reshape(n, d)
+    nrm = heatL(t, np.linalg.norm(x, axis=1)) / heatL(t, np.ones(n))
+    np.testing.assert_allclose(np.linalg.norm(out, axis=1), nrm, rtol=1e-10)
+    cosang = np.einsum("ni,ni->n", out, hc) / (np.linalg.norm(out, axis=1) * np.linalg.norm(hc, axis=1))
+    np.testing.assert_allclose(cosang, 1.0, atol=1e-12)
+    np.testing.assert_allclose(cpu.vector_diffusion(x, 1e-13, Lc, cpu.laplacian(rp, cols)), x,
+                               atol=1e-11)
+
+
+def test_fit_lbfgs_returns_a_stationary_minimum():
+    """fit_lbfgs (main.py:87-95 via scipy L-BFGS-B): at the returned
+    hyperparameters the gradient with respect to every parameter off its
+    lower bound is ~0 against the gradient at the start, and no coordinate
+    step of +-1e-3 (relative) lowers the NLML."""
+    X = sphere(250, 6)
+    o = cpu.Data(X, n_neighbors=10, n_eigenpairs=20, dense_eigh=True)
+    rng = np.random.default_rng(7)
+    nodes = rng.choice(o.n, size=120, replace=False)
+    o.random_vector_field(seed=3)
+    o.smooth_vector_field(t=5, method="eigh")
+    k = o.evecs_Lc.shape[1]
+    Phi = o.evecs_Lc.reshape(o.n, -1)[nodes].reshape(-1, k)
+    y = o.vectors[nodes].reshape(-1, 1) + 0.01 * rng.normal(size=(len(nodes) * 3, 1))
+    th, res = cpu.fit_lbfgs(Phi, y, o.evals_Lc, o.n * 3, maxiter=500)
+    assert res.success, res.message
+
+    def f_and_g(t):
+        S, dS = cpu.spectral_filter(o.evals_Lc, t[0], t[1], t[2], o.n * 3)
+        gS, gn = cpu.gpr_grad(Phi, y, S, t[3])
+        return cpu.gpr_nlml(Phi, y, S, t[3]), np.array([dS[0] @ gS, dS[1] @ gS, dS[2] @ gS, gn])
+
+    f0, g0 = f_and_g(np.array([1.5, 5.0, 1.0, 1.0]))
+    fs, gs = f_and_g(th)
+    assert fs < f0
+    lower = np.array([0.0, 0.0, 0.0, 1e-6])
+    free = th > lower * 1.01 + 1e-8
+    assert np.all(np.abs(gs[free] * th[free]) < 1e-3 * np.abs(g0[free] * np.array([1.5, 5.0, 1.0, 1.0])[free]) + 1e-4)
+    for i in range(4):
+        for s in (-1e-3, 1e-3):
+            tp = th.copy()
+            tp[i] *= 1 + s
+            if tp[i] <= lower[i]:
+                continue
+            assert f_and_g(tp)[0] >= fs - 1e-9 * abs(fs)

@@ -239,6 +239,17 @@ size_t rvgp_colsumsq_ws_bytes(int M, int k);
 int rvgp_colsumsq(int f64, int M, int k, const void* Z, int ldz, double* out, void* ws,
                   size_t ws_bytes, void* stream);
 
+/* rvgp_colsumsq without a workspace argument; no counterpart in the
+ * reference.  For callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc): the
+ * partial sums live in a fixed 8 MiB device array held by the library (one
+ * per GPU, until the process ends), so the call allocates nothing.  Limit:
+ * min(1024, ceil(M/32)) * k <= 2^20, else RVGP_EINVAL.  Results equal
+ * rvgp_colsumsq's bit for bit.  Calls that overlap on one GPU share the array
+ * and race; the caller orders them. */
+int rvgp_colsumsq_noalloc(int f64, int M, int k, const void* Z, int ldz, double* out,
+                          void* stream);
+
 /* ---------------------------------------------------------------------
  * a12/a14/a15  Spectral kernel, NLML and adjoint -- replaces
  *     ManifoldKernel.eval_S / K / K_diag (RVGP/kernels.py:41-67) and GPflow's

@@ -28,6 +28,8 @@
 // the unconstrained parameters u of theta = softplus(u) + lower.
 #include "common.h"
 
+#include <mutex>
+
 namespace rvgp {
 
 constexpr int FT = 256;
@@ -362,11 +364,8 @@ extern "C" size_t rvgp_colsumsq_ws_bytes(int M, int k) {
     return sizeof(double) * (size_t)colsumsq_parts(M) * (k > 0 ? k : 0);
 }
 
-extern "C" int rvgp_colsumsq(int f64, int M, int k, const void* Z, int ldz, double* out, void* ws,
-                             size_t ws_bytes, void* stream) {
-    RVGP_CHECK_ARG(M >= 0 && k >= 1 && ldz >= k, "rvgp_colsumsq: bad sizes");
-    RVGP_CHECK_ARG(ws_bytes >= rvgp_colsumsq_ws_bytes(M, k), "rvgp_colsumsq: workspace too small");
-    double* part = static_cast<double*>(ws);
+static int colsumsq_launch(int f64, int M, int k, const void* Z, int ldz, double* out,
+                           double* part, void* stream) {
     const int parts = colsumsq_parts(M);
     dim3 g(cdiv(k, 256), parts);
     if (f64)
@@ -380,6 +379,61 @@ extern "C" int rvgp_colsumsq(int f64, int M, int k, const void* Z, int ldz, doub
                        parts, part, out);
     RVGP_LAUNCH_CHECK();
     return RVGP_OK;
+}
+
+extern "C" int rvgp_colsumsq(int f64, int M, int k, const void* Z, int ldz, double* out, void* ws,
+                             size_t ws_bytes, void* stream) {
+    RVGP_CHECK_ARG(M >= 0 && k >= 1 && ldz >= k, "rvgp_colsumsq: bad sizes");
+    RVGP_CHECK_ARG(ws_bytes >= rvgp_colsumsq_ws_bytes(M, k), "rvgp_colsumsq: workspace too small");
+    return colsumsq_launch(f64, M, k, Z, ldz, out, static_cast<double*>(ws), stream);
+}
+
+// ---------------------------------------------------------------------------
+// rvgp_colsumsq without a workspace argument, for callers that cannot
+// allocate device memory (code under HIP graph capture, serving loops).  The
+// per-slice partial sums live in a fixed device array, `scratch`, of
+// kNoAllocScratchDoubles doubles (8 MiB), one copy per GPU that the library
+// runs on, held until the process ends.  Host code never passes the array's
+// name to a kernel (in host code it is only a stand-in for the symbol): the
+// device address of each GPU's copy is looked up once per device with
+// hipGetSymbolAddress and cached.  Every call on one GPU shares that GPU's
+// copy, so calls that overlap on one GPU (different streams, or host threads)
+// race and must be ordered by the caller.  The partial-sum kernel writes
+// every element the reduction reads, so nothing needs clearing between calls.
+constexpr size_t kNoAllocScratchDoubles = (size_t)1 << 20;
+static __device__ __align__(256) double scratch[kNoAllocScratchDoubles];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[kMaxDevices] = {};
+
+static int scratch_address(double** out) {
+    int dev = 0;
+    RVGP_CHECK_HIP(hipGetDevice(&dev));
+    RVGP_CHECK_ARG(dev >= 0 && dev < kMaxDevices, "scratch: device ordinal %d out of range", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[dev] == nullptr) {
+        void* p = nullptr;
+        RVGP_CHECK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        RVGP_CHECK_ARG(p != nullptr, "scratch: no device address for the scratch array");
+        g_scratch_ptr[dev] = static_cast<double*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return RVGP_OK;
+}
+
+extern "C" int rvgp_colsumsq_noalloc(int f64, int M, int k, const void* Z, int ldz, double* out,
+                                     void* stream) {
+    RVGP_CHECK_ARG(M >= 0 && k >= 1 && ldz >= k, "rvgp_colsumsq_noalloc: bad sizes");
+    const size_t need = (size_t)colsumsq_parts(M) * (size_t)k;
+    RVGP_CHECK_ARG(need <= kNoAllocScratchDoubles,
+                   "rvgp_colsumsq_noalloc: %d columns of %d rows need %zu scratch doubles, more "
+                   "than the %zu the library holds",
+                   k, M, need, kNoAllocScratchDoubles);
+    double* part = nullptr;
+    const int rc = scratch_address(&part);
+    if (rc != RVGP_OK) return rc;
+    return colsumsq_launch(f64, M, k, Z, ldz, out, part, stream);
 }
 
 extern "C" int rvgp_ws_finish(int k, double M, int nrhs, const double* L, int ldl,

@@ -234,6 +234,17 @@ int rvgp_trsm(int f64, int trans, int n, int r, const void* L, int lda, void* B,
 /* out[0] = sum_i log(L_ii) accumulated in f64 (half the log-determinant). */
 int rvgp_chol_logdet(int f64, int n, const void* L, int lda, double* out, void* stream);
 
+/* Symmetric eigendecomposition A = V diag(w) V^T in FP64, w ascending, the
+ * eigenvectors in the columns of V (n x n, row stride ldv): the dense
+ * eigenproblem of the Rayleigh-Ritz step that ARPACK solves inside eigsh
+ * (geometry.py:73).  Householder tridiagonalisation, multisection bisection,
+ * inverse iteration (Gram-Schmidt inside eigenvalue clusters), blocked
+ * back-transformation.  A is not modified.  1 <= n <= 2048; workspace
+ * rvgp_syev_ws_bytes(n) bytes. */
+size_t rvgp_syev_ws_bytes(int n);
+int rvgp_syev(int n, double* A, int lda, double* w, double* V, int ldv, void* ws,
+              size_t ws_bytes, void* stream);
+
 /* out[j] = sum_i Z[i][j]^2 (f64 accumulation, fixed order); Z (M, k) f64/f32. */
 size_t rvgp_colsumsq_ws_bytes(int M, int k);
 int rvgp_colsumsq(int f64, int M, int k, const void* Z, int ldz, double* out, void* ws,

@@ -46,6 +46,8 @@ SIGNATURES = {
     "rvgp_diffusion_normalise": (I, [S, I, P, I, P, I, P, I, P]),
     "rvgp_phi_rows": (I, [I, I, I, I, I, P, I, I, P, I, P, P, I, P]),
     "rvgp_gemm_sym": (I, [I, I, I, P, I, P, I, P, I, I, P, S, P]),
+    "rvgp_syev_ws_bytes": (S, [I]),
+    "rvgp_syev": (I, [I, P, I, P, P, I, P, S, P]),
     "rvgp_colsumsq_ws_bytes": (S, [I, I]),
     "rvgp_colsumsq": (I, [I, I, I, P, I, P, P, S, P]),
     "rvgp_colsumsq_noalloc": (I, [I, I, I, P, I, P, P]),

@@ -129,3 +129,22 @@ def chol_logdet_half(L):
     _lib.call("rvgp_chol_logdet", int(L.dtype == torch.float64), L.shape[0], _lib.ptr(L), _ld(L),
               _lib.ptr(out), _lib.stream())
     return out
+
+
+def syev(A):
+    """Eigenvalues (ascending) and eigenvectors (columns) of the symmetric
+    FP64 matrix A on the device (rvgp_syev), as torch.linalg.eigh returns
+    them.  A is not modified."""
+    n = A.shape[0]
+    assert A.shape == (n, n) and A.dtype == torch.float64
+    work = A.contiguous()
+    w = torch.empty(n, dtype=torch.float64, device=A.device)
+    V = torch.empty((n, n), dtype=torch.float64, device=A.device)
+    ws_bytes = _lib.lib().rvgp_syev_ws_bytes(n)
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=A.device)
+    ev = instrument.begin()
+    _lib.call("rvgp_syev", n, _lib.ptr(work), n, _lib.ptr(w), _lib.ptr(V), n, _lib.ptr(ws),
+              ws_bytes, _lib.stream())
+    if ev is not None:
+        instrument.end(ev, "syev", "f64", 8 * 3 * n * n, 4.0 * n ** 3 / 3.0 + 2.0 * n ** 3)
+    return w, V

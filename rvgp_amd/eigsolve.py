@@ -21,7 +21,9 @@ precision="mixed": the filter and the tall products run on FP32 blocks with
 the Grams accumulated in FP64 (f32 -> f64 MFMA); once the FP32 residuals
 stall at FP32 level, FP64 polish iterations (FP64 SpMM + FP64 RR) bring the
 wanted pairs to the FP64 tolerance.  The small projected problems (p x p,
-p = k + guard) are solved on the GPU in FP64 (torch.linalg.eigh).
+p = k + guard) are solved on the GPU in FP64 by the library's own symmetric
+eigensolver (dense.syev: tridiagonalisation, bisection, inverse iteration),
+and the p x p products run on its matrix-core GEMM.
 
 Returns eigenvalues ascending and eigenvectors with unit 2-norm; the callers
 scale by sqrt(rows) as geometry.py:75 does.
@@ -63,14 +65,14 @@ def _rayleigh_ritz(A, Y, work_dtype):
     if Bm is None:
         # numerically dependent block: eigendecomposition of G drops the
         # directions below the rounding level instead of failing
-        g, V = torch.linalg.eigh(G)
+        g, V = dense.syev(G)
         gmax = float(g[-1].item())
         keep = g > gmax * drop
-        Bm = V[:, keep] / torch.sqrt(g[keep])
-    Hb = Bm.T @ H @ Bm
+        Bm = (V[:, keep] / torch.sqrt(g[keep])).contiguous()
+    Hb = dense.gemm(Bm, dense.gemm(H, Bm), transA=True)
     Hb = 0.5 * (Hb + Hb.T)
-    theta, Q = torch.linalg.eigh(Hb)
-    Cm = (Bm @ Q).to(work_dtype).contiguous()
+    theta, Q = dense.syev(Hb)
+    Cm = dense.gemm(Bm, Q).to(work_dtype).contiguous()
     X = dense.gemm(Y, Cm)
     AX = dense.gemm(W, Cm)
     return X, theta, AX

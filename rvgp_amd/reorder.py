@@ -19,11 +19,12 @@ import torch
 def morton_perm(X, max_dims=6):
     """Permutation (new position -> old node) sorting nodes by the Morton
     code of their top-P principal coordinates (P = min(D, max_dims))."""
-    Xc = X - X.mean(0, keepdim=True)
-    C = Xc.T @ Xc
-    w, V = torch.linalg.eigh(C)
+    from . import dense
+    Xc = (X - X.mean(0, keepdim=True)).contiguous()
+    C = dense.gemm_sym(Xc, Xc)
+    w, V = dense.syev(C)
     P = min(X.shape[1], max_dims)
-    Y = Xc @ V[:, -P:]
+    Y = dense.gemm(Xc, V[:, -P:].contiguous())
     bits = min(20, 62 // P)
     lo = Y.min(0).values
     span = (Y.max(0).values - lo).clamp_min(1e-300)

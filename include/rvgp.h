@@ -90,6 +90,14 @@ size_t rvgp_knn_ws_bytes(int n, int k);
 int rvgp_knn(const double* x, int n, int D, int k, const int* qperm, int* nbr, void* ws,
              size_t ws_bytes, void* stream);
 
+/* Dense Gaussian affinity of manifold_graph(typ='affinity')
+ * (geometry.py:114-118): A (n x n, row stride lda) with
+ * A[i][j] = exp(-|x_i - x_j|^2 / (2 sigma^2)) (the reference fixes
+ * sigma = 0.1); x (n, D) f64, D <= 64.  The non-zero entries are the graph's
+ * weighted edges (self loops of weight 1 included), as nx.from_numpy_array
+ * reads them. */
+int rvgp_affinity(const double* x, int n, int D, double sigma, double* A, int lda, void* stream);
+
 /* Workspace bytes for rvgp_graph_symmetrize. */
 size_t rvgp_graph_symmetrize_ws_bytes(int n, int k);
 

@@ -11,6 +11,8 @@ tests/test_oracle.py:
                       direct-formula C loop; brute-force invariants.
   knn / sym_csr       geometry.py:100-112   pinned: sklearn ``kneighbors_graph``
                       + networkx ``adjacency_matrix`` on the same points.
+  affinity_graph      geometry.py:114-118   pinned: brute-force Gaussian
+                      weights on small inputs; symmetric, unit diagonal.
   geo_nbh             ptu_dijkstra.pyx:361-404  pinned: golden Sigma/tangents
                       written by the reference's own compiled module
                       (tests/golden/heap_*.npz).
@@ -30,6 +32,9 @@ tests/test_oracle.py:
                       e^{-tL}a along v); for any connection |out_i| =
                       (e^{-tL}|x|)_i / (e^{-tL}1)_i with out_i parallel to
                       (e^{-tLc}x)_i from Lc's own eigendecomposition; t -> 0.
+                      "spectral" (smoothing.py:17-32): over the full
+                      eigenbasis it equals "expm"; the sqrt(rows) scale of
+                      stored eigenvectors cancels in the normalisation.
   project_to_manifold geometry.py:165-168   pinned: projector identities
                       (P^2 = P, P T = T, residual orthogonal to T, |Px| <= |x|).
   express_in_local_frame geometry.py:171-175  pinned: T^T (T c) = c,
@@ -55,6 +60,7 @@ import subprocess
 
 import numpy as np
 import scipy.linalg
+import scipy.special
 import scipy.sparse as sp
 import scipy.sparse.linalg
 
@@ -160,6 +166,20 @@ def sym_csr(nbr):
     if nnz < 0:
         raise MemoryError("oracle_sym_csr")
     return rowptr, cols[:nnz].copy()
+
+
+def affinity_graph(x, sigma=0.1):
+    """manifold_graph(typ='affinity') (geometry.py:114-118): sklearn
+    pairwise_distances, the Gaussian affinity, nx.from_numpy_array; returns
+    the networkx graph and its weighted adjacency (scipy csr)."""
+    import networkx as nx
+    from sklearn.metrics import pairwise_distances
+    Dm = pairwise_distances(np.asarray(x, dtype=np.float64))
+    A = np.exp(-Dm ** 2 / (2 * sigma ** 2))
+    G = nx.from_numpy_array(A)
+    W = sp.csr_matrix(nx.adjacency_matrix(G, weight="weight"), dtype=np.float64)
+    W.sort_indices()
+    return G, W
 
 
 def graph_csr_networkx(x, n_neighbors):
@@ -375,9 +395,23 @@ def heat(A, t, x, method="expm"):
     return V @ (np.exp(-t * w)[:, None] * (V.T @ x))
 
 
+def heat_spectral(evals, evecs, t, x):
+    """smoothing.py:17-32 (the "spectral" method, as the reference means it):
+    evecs diag(exp(-t evals)) evecs^T x."""
+    return evecs @ (np.exp(-t * np.asarray(evals))[:, None] * (evecs.T @ x))
+
+
 def vector_diffusion(x, t, Lc, L, method="expm"):
-    """smoothing.py:37-64 with normalise=True."""
+    """smoothing.py:37-64 with normalise=True.  method "expm" / "eigh": Lc, L
+    operators; "spectral": Lc, L = (eigenvalues, eigenvectors) pairs."""
     n, d = x.shape
+    if method == "spectral":
+        rows = np.asarray(Lc[1]).shape[0]
+        out = heat_spectral(Lc[0], Lc[1], t, x.reshape(rows, -1)).reshape(x.shape)
+        x_abs = np.linalg.norm(x, axis=-1, keepdims=True)
+        out_abs = heat_spectral(L[0], L[1], t, x_abs)
+        ind = heat_spectral(L[0], L[1], t, np.ones([n, 1]))
+        return out * out_abs / (ind * np.linalg.norm(out, axis=-1, keepdims=True))
     out = heat(Lc, t, x.reshape(n * d, -1), method).reshape(x.shape)
     x_abs = np.linalg.norm(x, axis=-1, keepdims=True)
     out_abs = heat(L, t, x_abs, method)
@@ -589,7 +623,7 @@ def fit_lbfgs(Phi, y, evals, n_rows, theta0=(1.5, 5.0, 1.0, 1.0), lower=(0.0, 0.
         val = gpr_nlml(Phi, y, S, th[3])
         gS, gn = gpr_grad(Phi, y, S, th[3])
         g = np.array([dS[0] @ gS, dS[1] @ gS, dS[2] @ gS, gn])
-        return val, g * (1.0 / (1.0 + np.exp(-u)))
+        return val, g * scipy.special.expit(u)   # d softplus / du, overflow-free
 
     res = scipy.optimize.minimize(f, u0, jac=True, method="L-BFGS-B", options={"maxiter": maxiter})
     return softplus(res.x) + lower, res

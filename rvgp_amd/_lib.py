@@ -29,6 +29,7 @@ SIGNATURES = {
     "rvgp_fps": (I, [P, I, I, I, I, D, I, P, P, P, P, P, P]),
     "rvgp_knn_ws_bytes": (S, [I, I]),
     "rvgp_knn": (I, [P, I, I, I, P, P, P, S, P]),
+    "rvgp_affinity": (I, [P, I, I, D, P, I, P]),
     "rvgp_graph_symmetrize_ws_bytes": (S, [I, I]),
     "rvgp_graph_symmetrize": (I, [P, I, I, P, P, P, S, P]),
     "rvgp_geo_nbh_ws_bytes": (S, [I, I, I]),

@@ -375,9 +375,41 @@ static int knn_slices(int n) {
     return n <= 400000 ? 8 : n <= 800000 ? 4 : n <= 1600000 ? 2 : 1;
 }
 
+// Dense Gaussian affinity of geometry.manifold_graph(typ='affinity')
+// (geometry.py:114-118): A[i][j] = exp(-|x_i - x_j|^2 / (2 sigma^2)), the
+// squared distance in the fixed operation order of d2_direct.  16 x 16 tiles,
+// the two point blocks staged in LDS.
+__global__ __launch_bounds__(256) void affinity_kernel(const double* __restrict__ x, int n, int D,
+                                                       double inv2s2, double* __restrict__ A,
+                                                       int lda) {
+    __shared__ double xi[16][65], xj[16][65];
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    const int i0 = blockIdx.y * 16, j0 = blockIdx.x * 16;
+    for (int e = threadIdx.x; e < 16 * D; e += 256) {
+        const int r = e / D, c = e - r * D;
+        xi[r][c] = i0 + r < n ? x[(size_t)(i0 + r) * D + c] : 0.0;
+        xj[r][c] = j0 + r < n ? x[(size_t)(j0 + r) * D + c] : 0.0;
+    }
+    __syncthreads();
+    const int i = i0 + ty, j = j0 + tx;
+    if (i >= n || j >= n) return;
+    const double d2 = d2_direct(&xi[ty][0], &xj[tx][0], D);
+    A[(size_t)i * lda + j] = exp(-d2 * inv2s2);
+}
+
 }  // namespace rvgp
 
 using namespace rvgp;
+
+extern "C" int rvgp_affinity(const double* x, int n, int D, double sigma, double* A, int lda,
+                             void* stream) {
+    RVGP_CHECK_ARG(n >= 1 && D >= 1 && D <= 64 && lda >= n, "rvgp_affinity: need D <= 64, lda >= n");
+    RVGP_CHECK_ARG(sigma > 0.0, "rvgp_affinity: sigma must be positive");
+    hipLaunchKernelGGL(affinity_kernel, dim3(cdiv(n, 16), cdiv(n, 16)), dim3(256), 0,
+                       as_stream(stream), x, n, D, 1.0 / (2.0 * sigma * sigma), A, lda);
+    RVGP_LAUNCH_CHECK();
+    return RVGP_OK;
+}
 
 extern "C" size_t rvgp_knn_ws_bytes(int n, int k) {
     if (n <= 0 || k <= 0) return 0;

@@ -19,7 +19,7 @@ from . import _lib, geometry
 from .eigsolve import chfsi
 from .ptu_dijkstra import connections_device, tangent_frames_device
 from .reorder import morton_perm, permute_bsr, unpermute_rows
-from .smoothing import vector_diffusion_device
+from .smoothing import lambda_min_lower, vector_diffusion_device, vector_diffusion_spectral_device
 from .sparse import BSR, graph_laplacian
 
 
@@ -303,13 +303,27 @@ class data:
         vd = vd / torch.linalg.vector_norm(vd, dim=1, keepdim=True)
         self.vectors = vd
 
-    def smooth_vector_field(self, t=100):
+    def smooth_vector_field(self, t=100, method="matrix_exp"):
         """Vector diffusion smoothing (dataclass.py:105-120).  vectors may be
-        (n, D) or a batch (F, n, D) of fields smoothed in one pass."""
+        (n, D) or a batch (F, n, D) of fields smoothed in one pass.
+        method="matrix_exp" (the reference's) applies exp(-t Lc) and exp(-t L)
+        to the operators; method="spectral" (smoothing.py:17-32, an option
+        the reference's data object does not expose) uses the data object's
+        own eigenpairs -- the heat kernel restricted to the computed spectrum."""
         if self._vectors is None:
             raise TypeError("No vectors to smooth: data.vectors is None")
         c = self._frame(0, self._vectors, self.dim_man)
-        c = vector_diffusion_device(c, t, self._Lcb, self._Lb)
+        if method == "spectral":
+            if c.dim() != 2 or np.ndim(t) != 0:
+                raise ValueError("method='spectral' smooths one field with one diffusion time")
+            c = vector_diffusion_spectral_device(c, float(t), self.spectrum_device(), self._U,
+                                                 self._evals_L, self._evecs_L)
+        elif method == "matrix_exp":
+            st = self.eig_stats["Lc"]
+            lc_min = lambda_min_lower(self._Lcb, float(self._evals_Lc[0].item()), st["residual"])
+            c = vector_diffusion_device(c, t, self._Lcb, self._Lb, lc_min=lc_min)
+        else:
+            raise NotImplementedError
         self.vectors = self._frame(1, c, self._X.shape[1])
 
 

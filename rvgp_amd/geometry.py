@@ -103,6 +103,29 @@ def symmetrize_device(nbr):
     return rowptr, cols[:nnz].clone()
 
 
+AFFINITY_SIGMA = 0.1  # geometry.py:116
+
+
+def affinity_graph(X, sigma=AFFINITY_SIGMA):
+    """manifold_graph(typ='affinity') (geometry.py:114-118): the complete
+    graph weighted by the Gaussian affinity exp(-|x_i - x_j|^2 / (2 sigma^2))
+    of all pairs, self loops (weight 1) included, entries that underflow to 0
+    left out (nx.from_numpy_array's rule).  The dense affinity is formed on the
+    device (rvgp_affinity); the result is a weighted :class:`SymGraph` whose
+    networkx view carries the weights and the 'pos' attribute."""
+    xd = _dev_f64(X)
+    n, D = xd.shape
+    A = torch.empty((n, n), dtype=torch.float64, device=xd.device)
+    _lib.call("rvgp_affinity", _lib.ptr(xd), n, D, float(sigma), _lib.ptr(A), n, _lib.stream())
+    nz = A != 0
+    rowptr = torch.zeros(n + 1, dtype=torch.int64, device=xd.device)
+    rowptr[1:] = torch.cumsum(nz.sum(1), 0)
+    cols = torch.nonzero(nz)[:, 1].to(torch.int32).contiguous()
+    g = SymGraph(rowptr.to(torch.int32), cols, n, positions=X if isinstance(X, np.ndarray) else None)
+    g._weights = A[nz].contiguous()
+    return g
+
+
 KNN_ORDER_MIN_N = 4096  # below this the kNN scan is too short for a query order to pay
 
 
@@ -115,8 +138,10 @@ def manifold_graph(X, typ='knn', n_neighbors=5, query_order=None):
     coordinates from n = 4096 on) groups manifold neighbours into the kNN's
     waves; the graph does not depend on it.
     """
+    if typ == 'affinity':
+        return affinity_graph(X)
     if typ != 'knn':
-        raise NotImplementedError("typ='affinity' (geometry.py:114-118) is not on the GPU path")
+        raise UnboundLocalError("local variable 'G' referenced before assignment")
     xd = _dev_f64(X)
     if query_order is None and xd.shape[0] >= KNN_ORDER_MIN_N:
         from .reorder import morton_perm

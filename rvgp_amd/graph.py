@@ -4,8 +4,9 @@ The reference builds a networkx graph (RVGP/geometry.py:100-123) and every
 consumer converts it back to a symmetrised CSR matrix
 (RVGP/lib/ptu_dijkstra.pyx:84-103).  Here the graph *is* that CSR, kept in
 HBM: ``rowptr`` (n+1) and ``cols`` (nnz) int32, columns sorted within each
-row, one self loop per node, all edge weights 1.0.  A networkx view is built
-on the host only when someone asks for it (``to_networkx``).
+row, one self loop per node, edge weights 1.0 (kNN) or per-edge values
+(the affinity graph).  A networkx view is built on the host only when someone
+asks for it (``to_networkx``).
 """
 import numpy as np
 import torch
@@ -41,11 +42,13 @@ class SymGraph:
     def to_scipy(self):
         import scipy.sparse as sp
         rp, c = self.csr_numpy()
-        return sp.csr_matrix((np.ones(len(c)), c, rp), shape=(self.n, self.n))
+        w = np.ones(len(c)) if self._weights is None else self._weights.cpu().numpy()
+        return sp.csr_matrix((w, c, rp), shape=(self.n, self.n))
 
     def to_networkx(self):
         """The networkx graph the reference returns from manifold_graph:
-        undirected, weight 1.0, self loops, node attribute 'pos'."""
+        undirected, edge weights (1.0 for the kNN graph, the affinities for
+        typ='affinity'), self loops, node attribute 'pos'."""
         if self._nx is None:
             import networkx as nx
             G = nx.from_scipy_sparse_array(self.to_scipy())

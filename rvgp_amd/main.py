@@ -110,8 +110,15 @@ class manifold_GPR:
             # a Python bool list indexes as a mask in numpy
             mean, var = self.predict_nodes(data, checked_nodes(test_ind, data.n))
         elif isinstance(first, floats):
+            # positional encodings: spectral rows (k values each) given
+            # directly (main.py:108-109).  The reference then reshapes to
+            # len(test_ind) rows -- the number of floats, which cannot hold
+            # the predictions -- so here each spectral row gets one row of
+            # output: mean (rows, R) and variance (rows, 1).
             test_x = np.asarray(test_ind, dtype=np.float64).reshape(-1, self.engine.k)
             mean, var = self.predict_f(test_x)
+            rows = test_x.shape[0]
+            return mean.cpu().numpy().reshape(rows, -1), var.cpu().numpy().reshape(rows, -1)
         else:  # np.int64 / np.bool_ match neither branch of main.py:104-109
             raise UnboundLocalError("local variable 'test_x' referenced before assignment")
         if self.engine.R > 1:

@@ -37,13 +37,24 @@ def cheb_coeffs(t, ub, rtol=1e-17, max_terms=200000):
     return c[:max(J, 2)]
 
 
-def expm_multiply(A: BSR, X, t, ub=None):
-    """exp(-t A) X for a PSD BSR operator A (X: (rows,) or (rows, r)).  t is
-    a scalar, or one diffusion time per column of X: the recurrence T_j X
-    then runs once, to the longest series, and each column accumulates its
-    own coefficients."""
+def expm_multiply(A: BSR, X, t, ub=None, shift=0.0):
+    """exp(-t (A - shift I)) X for a PSD BSR operator A (X: (rows,) or
+    (rows, r)); shift must not exceed A's smallest eigenvalue.  t is a
+    scalar, or one diffusion time per column of X: the recurrence T_j X then
+    runs once, to the longest series, and each column accumulates its own
+    coefficients.
+
+    The series' error is absolute -- about 1e-17 of |X| -- so when exp(-t A)
+    X is itself tiny (a large t over a spectrum bounded away from 0, e.g. a
+    connection Laplacian on a curved surface) its relative accuracy is lost;
+    with shift = lambda_min the result keeps its full relative accuracy and
+    exp(-t shift) is the caller's (normalised diffusion divides it out)."""
     if ub is None:
         ub = A.spectral_upper_bound() * 1.0001 + 1e-12
+    shift = float(shift)
+    if shift > 0.0:
+        # the series of exp(-t lambda') on [0, ub - shift] in lambda' = lambda - shift
+        return _expm_multiply_shifted(A, X, t, ub, shift)
     X2 = X.reshape(-1, 1) if X.dim() == 1 else X
     ts = np.atleast_1d(np.asarray(t, dtype=np.float64))
     if ts.size > 1:
@@ -77,11 +88,53 @@ def expm_multiply(A: BSR, X, t, ub=None):
     return out.reshape(X.shape)
 
 
-def vector_diffusion_device(c, t, Lc: BSR, L: BSR):
+def _expm_multiply_shifted(A, X, t, ub, shift):
+    ubs = ub - shift
+    X2 = X.reshape(-1, 1) if X.dim() == 1 else X
+    ts = np.atleast_1d(np.asarray(t, dtype=np.float64))
+    cs = [cheb_coeffs(tc, ubs) for tc in ts]
+    C = np.zeros((max(len(c) for c in cs), X2.shape[1]))
+    for col in range(X2.shape[1]):
+        c = cs[col if ts.size > 1 else 0]
+        C[:len(c), col] = c
+    Cd = torch.as_tensor(C, dtype=X2.dtype, device=X2.device)
+    # T_1 = (2/ubs)(A - shift) X - X ; T_j = (4/ubs)(A - shift) T_{j-1} - 2 T_{j-1} - T_{j-2}
+    out = X2 * Cd[0]
+    if C.shape[0] > 1:
+        T1 = spmm(A, X2, alpha=2.0 / ubs, beta=-1.0 - 2.0 * shift / ubs)
+        out.addcmul_(T1, Cd[1])
+        prev, cur = X2.clone(), T1
+        for j in range(2, C.shape[0]):
+            spmm(A, cur, Y=prev, alpha=4.0 / ubs, beta=-2.0 - 4.0 * shift / ubs, Z=prev,
+                 gamma=-1.0)
+            out.addcmul_(prev, Cd[j])
+            prev, cur = cur, prev
+    return out.reshape(X.shape)
+
+
+def lambda_min_lower(A: BSR, evals=None, residual=None):
+    """A value at or below A's smallest eigenvalue, for the heat-kernel
+    shift: from a computed smallest eigenvalue and its residual bound
+    (|theta - lambda| <= |A x - theta x|), or from a one-pair eigensolve."""
+    scale = A.gershgorin_bound()
+    if evals is None:
+        from .eigsolve import chfsi
+        w, _, st = chfsi(A, 1, precision="fp64")
+        theta, res = float(w[0].item()), float(st["residual"]) * scale
+    else:
+        theta, res = float(evals), float(residual) * scale
+    return max(0.0, theta - 2.0 * res - 1e-12 * scale)
+
+
+def vector_diffusion_device(c, t, Lc: BSR, L: BSR, lc_min=0.0):
     """smoothing.py:37-64 (method="matrix_exp", normalise=True) on device.
 
     c: (n, d) local-frame coefficients, or (F, n, d) for F fields at once;
-    t: a diffusion time, or (F,) times, one per field."""
+    t: a diffusion time, or (F,) times, one per field.  lc_min: a lower bound
+    of Lc's smallest eigenvalue; the connection diffusion runs shifted by it
+    (exp(-t lc_min) is common to every node and cancels in the
+    normalisation), which keeps the smoothed directions accurate however
+    small exp(-t Lc) c is."""
     batched = c.dim() == 3
     C = c if batched else c.unsqueeze(0)
     F, n, d = C.shape
@@ -91,7 +144,8 @@ def vector_diffusion_device(c, t, Lc: BSR, L: BSR):
         raise ValueError(f"{ts.size} diffusion times for {F} fields")
     # connection diffusion of every field: columns = fields
     Xc = C.reshape(F, n * d).T.contiguous()
-    out = expm_multiply(Lc, Xc, ts if per_field else float(ts[0])).T.reshape(F, n, d).contiguous()
+    out = expm_multiply(Lc, Xc, ts if per_field else float(ts[0]),
+                        shift=lc_min).T.reshape(F, n, d).contiguous()
     # scalar diffusions exp(-tL)|x| and exp(-tL)1 share one multi-column pass
     # (one column of ones per distinct time)
     n1 = F if per_field else 1
@@ -106,27 +160,79 @@ def vector_diffusion_device(c, t, Lc: BSR, L: BSR):
     return out if batched else out[0]
 
 
+def spectral_diffusion_device(X, t, evals, evecs):
+    """evecs diag(exp(-t evals)) evecs^T X on the device (smoothing.py:17-32:
+    the heat kernel restricted to the given eigenpairs, eigenvectors taken as
+    they are passed).  X: (rows,) or (rows, r)."""
+    from . import dense
+    X2 = X.reshape(-1, 1) if X.dim() == 1 else X
+    V = evecs.contiguous()
+    C = dense.gemm(V, X2.contiguous(), transA=True)                 # (k, r)
+    C = C * torch.exp(-float(t) * evals.to(torch.float64)).reshape(-1, 1)
+    return dense.gemm(V, C.contiguous()).reshape(X.shape)
+
+
+def vector_diffusion_spectral_device(c, t, evals_c, evecs_c, evals_l, evecs_l):
+    """smoothing.py:37-64 with method="spectral" and normalise=True on the
+    device: the connection heat kernel and the scalar ones (for |c| and 1)
+    from eigenpairs.  With normalisation any common scale of the eigenvectors
+    (the sqrt(rows) of geometry.py:75) cancels."""
+    n, d = c.shape
+    out = spectral_diffusion_device(c.reshape(-1, 1), t, evals_c, evecs_c).reshape(n, d)
+    xs = torch.empty((n, 2), dtype=torch.float64, device=c.device)
+    xs[:, 0] = torch.linalg.vector_norm(c, dim=1)
+    xs[:, 1] = 1.0
+    sd = spectral_diffusion_device(xs, t, evals_l, evecs_l)
+    return out * sd[:, :1] / (sd[:, 1:] * torch.linalg.vector_norm(out, dim=1, keepdim=True))
+
+
+def _host_eigenpairs(par):
+    assert isinstance(par, (list, tuple)) and len(par) == 2, \
+        "For spectral method, par must be a tuple of eigenvalues, eigenvectors!"
+    dev = _lib.device()
+    return (torch.as_tensor(np.asarray(par[0], dtype=np.float64), device=dev),
+            torch.as_tensor(np.asarray(par[1], dtype=np.float64), device=dev))
+
+
 def scalar_diffusion(x, t, method="matrix_exp", par=None):
-    """smoothing.py:8-34 for a host scipy operator ``par`` (method
-    "matrix_exp"); returns a numpy array."""
-    if method != "matrix_exp":
-        raise NotImplementedError("only method='matrix_exp' is on the RVGP path")
-    A = BSR.from_scipy(par, 1)
+    """smoothing.py:8-34; returns a numpy array.  "matrix_exp": exp(-t par) x
+    for a host scipy operator par; "spectral": par = (eigenvalues,
+    eigenvectors), evecs diag(exp(-t evals)) evecs^T x."""
     X = torch.as_tensor(np.asarray(x, dtype=np.float64), device=_lib.device())
-    return expm_multiply(A, X, t).cpu().numpy()
+    if method == "matrix_exp":
+        A = BSR.from_scipy(par, 1)
+        return expm_multiply(A, X, t).cpu().numpy()
+    if method == "spectral":
+        evals, evecs = _host_eigenpairs(par)
+        return spectral_diffusion_device(X, t, evals, evecs).cpu().numpy()
+    raise NotImplementedError
 
 
-def vector_diffusion(x, t, Lc, L=None, method="matrix_exp", normalise=True):
-    """smoothing.py:37-64 for host scipy operators; returns numpy (n, d)."""
-    if method != "matrix_exp":
-        raise NotImplementedError("only method='matrix_exp' is on the RVGP path")
+def vector_diffusion(x, t, Lc, L=None, method="spectral", normalise=True):
+    """smoothing.py:37-64; returns numpy (n, d).  method "matrix_exp": Lc, L
+    host scipy operators; "spectral": Lc = (evals_Lc, evecs_Lc) and
+    L = (evals_L, evecs_L) eigenpairs, e.g. a data object's."""
     n, d = x.shape[0], x.shape[1]
+    dev = _lib.device()
+    xd = torch.as_tensor(np.asarray(x, dtype=np.float64), device=dev)
+    if method == "spectral":
+        assert len(Lc) == 2, "Lc must be a tuple of eigenvalues, eigenvectors!"
+        nd = np.asarray(Lc[1]).shape[0]
+        assert (n * d % nd) == 0, "Data dimension must be an integer multiple of the dimensions of the connection Laplacian!"
+        ec, vc = _host_eigenpairs(Lc)
+        if not normalise:
+            return spectral_diffusion_device(xd.reshape(nd, -1), t, ec, vc).reshape(x.shape).cpu().numpy()
+        assert L is not None, 'Need Laplacian for normalised diffusion!'
+        el, vl = _host_eigenpairs(L)
+        return vector_diffusion_spectral_device(xd.reshape(n, -1), t, ec, vc, el, vl) \
+            .reshape(x.shape).cpu().numpy()
+    if method != "matrix_exp":
+        raise NotImplementedError
     nd = Lc.shape[0]
     assert (n * d % nd) == 0, "Data dimension must be an integer multiple of the dimensions of the connection Laplacian!"
-    dev = _lib.device()
     Lcb = BSR.from_scipy(Lc, nd // n if nd % n == 0 else 1)
-    xd = torch.as_tensor(np.asarray(x, dtype=np.float64), device=dev)
     if not normalise:
         return expm_multiply(Lcb, xd.reshape(nd, -1), t).reshape(x.shape).cpu().numpy()
     assert L is not None, 'Need Laplacian for normalised diffusion!'
-    return vector_diffusion_device(xd, t, Lcb, BSR.from_scipy(L, 1)).cpu().numpy()
+    return vector_diffusion_device(xd, t, Lcb, BSR.from_scipy(L, 1),
+                                   lc_min=lambda_min_lower(Lcb)).cpu().numpy()

@@ -256,3 +256,37 @@ def test_spectral_upper_bound_brackets_lambda_max():
         ub = A.spectral_upper_bound()
         assert lmax <= ub <= A.gershgorin_bound() + 1e-12
         assert ub <= 1.03 * lmax, (ub, lmax)
+
+
+def test_spectral_smoothing_vs_oracle():
+    """smooth_vector_field(method='spectral') and the drop-in
+    smoothing.vector_diffusion(method='spectral') (smoothing.py:17-64): the
+    heat kernel restricted to the computed eigenpairs, against the oracle's
+    restatement on its own eigenpairs (cut at a spectral gap, so the
+    truncated kernel does not depend on the basis); gauge-invariant ambient
+    output."""
+    import rvgp_amd as RVGP
+    from rvgp_amd import smoothing
+    X = sphere(500, 8)
+    o = cpu.Data(X, n_neighbors=10, n_eigenpairs=80, dense_eigh=True)
+    w, wl = o.evals_Lc, o.evals_L
+    # a cut where both spectra have a gap (the sphere's L has near-multiplets)
+    gaps = np.minimum(np.diff(w[40:71]) / w[41:71], np.diff(wl[40:71]) / wl[41:71])
+    k = 41 + int(np.argmax(gaps))
+    kl = 15 + int(np.argmax(np.diff(wl[15:40])))
+    d = RVGP.create_data_object(X, n_neighbors=10, n_eigenpairs=k, verbose=False)
+    d.random_vector_field(seed=2)
+    o.random_vector_field(seed=2)
+    v0 = o.vectors.copy()
+    # the data object's own spectra: k for Lc, and k for L
+    d.smooth_vector_field(t=3.0, method="spectral")
+    c = cpu.express_in_local_frame(v0, o.gauges)
+    U, Ul = o.U[:, :k], o.evecs_L[:, :k]
+    ref = cpu.vector_diffusion(c, 3.0, (w[:k], U), (wl[:k], Ul), method="spectral")
+    ref = cpu.express_in_local_frame(ref, o.gauges, reverse=True)
+    assert wl[k] - wl[k - 1] > 1e-3 * wl[k] and w[k] - w[k - 1] > 1e-3 * w[k]
+    np.testing.assert_allclose(d.vectors, ref, rtol=0, atol=1e-8 * np.abs(ref).max())
+    # drop-in function on host eigenpairs (ambient rows and the scalar pairs)
+    out = smoothing.vector_diffusion(c, 3.0, (w[:k], U), (wl[:kl], o.evecs_L[:, :kl]))
+    ref2 = cpu.vector_diffusion(c, 3.0, (w[:k], U), (wl[:kl], o.evecs_L[:, :kl]), method="spectral")
+    np.testing.assert_allclose(out, ref2, rtol=1e-10, atol=1e-12)

@@ -370,3 +370,33 @@ def test_rw_connection_laplacian_uses_networkx_degrees():
     Nref = nx.normalized_laplacian_matrix(Gx).toarray()
     for G in (Gx, Gd):
         np.testing.assert_allclose(geometry.compute_laplacian(G, True).toarray(), Nref, atol=1e-15)
+
+
+def test_affinity_graph_matches_oracle():
+    """manifold_graph(typ='affinity') (geometry.py:114-118): the Gaussian
+    affinity of all pairs (sigma 0.1) as a weighted graph, against the
+    oracle's sklearn pairwise_distances + nx.from_numpy_array; the heap
+    neighbourhoods over those weights equal the oracle's."""
+    import torch
+    from rvgp_amd import geometry, ptu_dijkstra
+    from rvgp_amd.graph import from_networkx
+    X = sphere(300, 4) * 0.3
+    g = geometry.manifold_graph(X, typ='affinity')
+    Go, Wo = cpu.affinity_graph(X)
+    W = g.to_scipy()
+    assert W.nnz == Wo.nnz
+    np.testing.assert_array_equal(W.indptr, Wo.indptr)
+    np.testing.assert_array_equal(W.indices, Wo.indices)
+    np.testing.assert_allclose(W.data, Wo.data, rtol=0, atol=1e-13)
+    G = g.to_networkx()
+    assert G.number_of_edges() == Go.number_of_edges()
+    assert np.allclose(G.nodes[5]["pos"], X[5])
+    # weighted Dijkstra neighbourhoods on the oracle's weights
+    go = from_networkx(Go)
+    K = 12
+    nbh = ptu_dijkstra.geo_nbh_device(go, K).cpu().numpy()
+    rp, c = Wo.indptr.astype(np.int32), Wo.indices.astype(np.int32)
+    ref, _ = cpu.geo_nbh(X, rp, c, K, weights=Wo.data, centred=False)
+    np.testing.assert_array_equal(nbh, ref)
+    with pytest.raises(UnboundLocalError):
+        geometry.manifold_graph(X, typ='radius')

@@ -504,3 +504,40 @@ def test_fit_lbfgs_returns_a_stationary_minimum():
             if tp[i] <= lower[i]:
                 continue
             assert f_and_g(tp)[0] >= fs - 1e-9 * abs(fs)
+
+
+def test_affinity_graph_oracle_is_the_gaussian_of_all_pairs():
+    """affinity_graph (geometry.py:114-118): every pair i, j (self loops
+    included) carries exp(-|x_i - x_j|^2 / (2 * 0.1^2)), brute force; the
+    graph is undirected and complete where no weight underflows."""
+    x = np.random.default_rng(2).normal(size=(40, 3)) * 0.15
+    G, W = cpu.affinity_graph(x)
+    d2 = ((x[:, None, :] - x[None, :, :]) ** 2).sum(-1)
+    ref = np.exp(-d2 / (2 * 0.1 ** 2))
+    np.testing.assert_allclose(W.toarray(), ref, rtol=1e-12, atol=1e-300)
+    assert G.number_of_edges() == 40 * 41 // 2
+    assert np.allclose(W.diagonal(), 1.0)
+
+
+def test_spectral_diffusion_oracle_equals_expm_on_the_full_basis():
+    """The "spectral" method (smoothing.py:17-32) with every eigenpair is the
+    dense heat kernel (== "expm"); a common scale of the eigenvectors (the
+    sqrt(rows) stored by compute_spectrum, geometry.py:75) cancels in the
+    normalised vector diffusion; a truncated basis only approximates it."""
+    X = sphere(150, 9)
+    rp, cols = cpu.sym_csr(cpu.knn(X, 8))
+    T, S = cpu.tangent_frames(X, rp, cols, 3, 12)
+    R = cpu.connections(np.ascontiguousarray(T[:, :, :2]), rp, cols)
+    L, Lc = cpu.laplacian(rp, cols), cpu.connection_laplacian(rp, cols, R)
+    wl, Vl = np.linalg.eigh(L.toarray())
+    wc, Vc = np.linalg.eigh(Lc.toarray())
+    x = np.random.default_rng(1).normal(size=(150, 2))
+    ref = cpu.vector_diffusion(x, 0.8, Lc, L, method="expm")
+    full = cpu.vector_diffusion(x, 0.8, (wc, Vc), (wl, Vl), method="spectral")
+    np.testing.assert_allclose(full, ref, atol=1e-10)
+    scaled = cpu.vector_diffusion(x, 0.8, (wc, Vc * np.sqrt(300)), (wl, Vl * np.sqrt(150)),
+                                  method="spectral")
+    np.testing.assert_allclose(scaled, full, atol=1e-12)
+    trunc = cpu.vector_diffusion(x, 0.8, (wc[:60], Vc[:, :60]), (wl[:30], Vl[:, :30]),
+                                 method="spectral")
+    assert np.abs(trunc - ref).max() > 1e-6

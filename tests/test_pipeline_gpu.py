@@ -197,3 +197,22 @@ def test_predict_nodes_equals_ambient_rows(pair, fields):
     m1, m2 = m1.cpu().numpy(), m2.cpu().numpy()
     np.testing.assert_allclose(m2, m1, rtol=1e-10, atol=1e-12 * np.abs(m1).max())
     np.testing.assert_allclose(v2.cpu().numpy(), v1.cpu().numpy(), rtol=1e-9, atol=1e-14)
+
+
+def test_transform_float_positional_encodings(pair):
+    """transform with floats (main.py:108-109): spectral rows given directly
+    (here the ambient rows of evecs_Lc of some nodes, flattened) predict the
+    same values as the int branch at those nodes, one row per spectral row."""
+    import rvgp_amd as RVGP
+    d, o, X = pair
+    d.random_vector_field(seed=5)
+    train_ind = np.random.default_rng(13).choice(np.arange(800), size=300, replace=False)
+    gp = RVGP.fit(d, train_ind=train_ind, epochs=15, verbose=False)
+    nodes = [3, 17, 400, 799]
+    m_int, v_int = gp.transform(d, nodes)
+    rows = d.evecs_Lc.reshape(800, -1)[nodes].reshape(-1, d.evecs_Lc.shape[1])
+    floats = [float(z) for z in rows.ravel()]
+    m_f, v_f = gp.transform(d, floats)
+    assert m_f.shape == (len(nodes) * 3, 1) and v_f.shape == (len(nodes) * 3, 1)
+    np.testing.assert_allclose(m_f.ravel(), m_int.ravel(), rtol=1e-10, atol=1e-13)
+    np.testing.assert_allclose(v_f.ravel(), v_int.ravel(), rtol=1e-9, atol=1e-13)

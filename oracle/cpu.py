@@ -472,7 +472,7 @@ def gpr_grad(Phi, y, S, noise):
     Ky = K + noise * np.eye(M)
     Kinv = np.linalg.inv(Ky)
     A = Kinv @ Y
-    PtKP = np.einsum("mj,mn,nj->j", Phi, Kinv, Phi)
+    PtKP = ((Kinv @ Phi) * Phi).sum(0)   # diag(Phi^T Ky^-1 Phi)
     PtA = Phi.T @ A
     gS = 0.5 * (R * PtKP - (PtA ** 2).sum(1))
     gn = 0.5 * (R * np.trace(Kinv) - (A * A).sum())

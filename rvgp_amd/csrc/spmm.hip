@@ -195,9 +195,7 @@ __global__ __launch_bounds__(256) void spmm_vec(
 // loads (wave-uniform addresses) straight into SGPRs that feed the FMAs, and
 // X is gathered with raw buffer loads: the per-lane column offset is a fixed
 // VGPR, the neighbour row's byte offset a scalar -- so per stored block a
-// lane issues B 16-byte loads and B*B*VW FMAs and nothing else.  Lanes past
-// the row's columns point their offset past the buffer, which the hardware
-// range check turns into a zero load without a memory access.  Needs X to
+// lane issues B 16-byte loads and B*B*VW FMAs and nothing else.  Needs X to
 // span < 2 GiB (32-bit offsets).
 constexpr int kBufRsrcWord3 = 0x00020000;  // raw 32-bit dwords, gfx9 family
 
@@ -216,10 +214,12 @@ __global__ __launch_bounds__(256) void spmm_buf(
     const int row = __builtin_amdgcn_readfirstlane(lw * WPB + (threadIdx.x >> 6));
     if (row >= nb) return;
     const int c = (blockIdx.y * gv + lane) * VW;
-    const bool on = lane < gv && c < r;
+    // lanes past the row's columns leave before the gathers: an exec-masked
+    // lane issues no address, so the wave's loads carry only live vectors
+    if (!(lane < gv && c < r)) return;
     const __amdgpu_buffer_rsrc_t xr =
         __builtin_amdgcn_make_buffer_rsrc((void*)X, (short)0, xbytes, kBufRsrcWord3);
-    const int voff = on ? c * (int)sizeof(T) : xbytes;
+    const int voff = c * (int)sizeof(T);
     const int rowbytes = ldx * (int)sizeof(T);
     V y[B];
 #pragma unroll
@@ -239,7 +239,6 @@ __global__ __launch_bounds__(256) void spmm_buf(
 #pragma unroll
             for (int bb = 0; bb < B; ++bb) y[a] += blk[a * B + bb] * xb[bb];
     }
-    if (!on) return;
 #pragma unroll
     for (int a = 0; a < B; ++a) {
         const size_t rr = (size_t)(row * B + a);

@@ -44,6 +44,7 @@ extern "C" {
 #define RVGP_ST_SHORT_COMPONENT 2  /* fewer than K+1 nodes reachable */
 #define RVGP_ST_POOL_OVERFLOW 3    /* heap node pool too small (host sizing bug) */
 #define RVGP_ST_NOT_SPD 4          /* Cholesky met a non-positive pivot */
+#define RVGP_ST_TIMEOUT 5          /* a bounded cross-workgroup wait gave up */
 
 /* Library version and last error message (thread-local, host). */
 int rvgp_version(void);
@@ -327,6 +328,32 @@ int rvgp_gp_params(const double* u, const double* lower, double* kernel_theta, d
 int rvgp_adam_step(double* u, const double* g, double* m, double* v, double* t,
                    const int* trainable, double lr, double b1, double b2, double eps,
                    void* stream);
+
+/* One whole weight-space optimiser step of a single fit in one launch
+ * (replaces, per step of the loop behind RVGP/main.py:87-95 -- GPflow's
+ * GPR.training_loss, its TF gradient and the optimiser update -- the
+ * sequence rvgp_gp_params, rvgp_spectral_filter, rvgp_ws_form, rvgp_potrf,
+ * the triangular inverse, rvgp_colsumsq, GEMVs, rvgp_ws_finish and
+ * rvgp_adam_step): from theta = softplus(u) + lower, the filter S of evals,
+ * A = G + s2 diag(S)^-1 (G: k x k, leading dimension ldg), its Cholesky
+ * factor and inverse, diag(A^-1), w = A^-1 b, and out (9) exactly as
+ * rvgp_ws_finish for one right-hand side (yy = y^T y, M observations);
+ * when do_adam, then the Adam update of u with state m1, m2 (4), t (1).
+ * A tile dataflow of ceil(k/32)^2 + 2 ceil(k/32) + 3 workgroups that wait on each other
+ * through flags in `sync` (rvgp_ws_step_sync_ints(k) ints): zero `sync`
+ * once before the first call and keep it for later calls on the same
+ * stream; calls that share a `sync` array must not overlap.  ws:
+ * rvgp_ws_step_ws_bytes(k) bytes.  1 <= k <= RVGP_WS_STEP_MAX_K.
+ * status[0]: RVGP_ST_OK, RVGP_ST_NOT_SPD, or RVGP_ST_TIMEOUT when a
+ * bounded wait gave up (the results of that call are then invalid). */
+#define RVGP_WS_STEP_MAX_K 2048
+size_t rvgp_ws_step_ws_bytes(int k);
+size_t rvgp_ws_step_sync_ints(int k);
+int rvgp_ws_step(int k, int typ, const double* evals, double n_rows, const double* G, int ldg,
+                 const double* b, const double* yy, double M, const double* lower, double* u,
+                 const int* trainable, double* m1, double* m2, double* t, int do_adam, double lr,
+                 double b1, double b2, double eps, void* ws, size_t ws_bytes, int* sync,
+                 size_t sync_ints, double* out, int* status, void* stream);
 
 /* ---------------------------------------------------------------------
  * Batched fits: F independent fields on one manifold, each with its own

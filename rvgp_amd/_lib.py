@@ -61,6 +61,10 @@ SIGNATURES = {
     "rvgp_lemma_terms": (I, [I, D, P, P, P, P, I, P, P, P]),
     "rvgp_gp_params": (I, [P, P, P, P, P]),
     "rvgp_adam_step": (I, [P, P, P, P, P, P, D, D, D, D, P]),
+    "rvgp_ws_step_ws_bytes": (S, [I]),
+    "rvgp_ws_step_sync_ints": (S, [I]),
+    "rvgp_ws_step": (I, [I, I, P, D, P, I, P, P, D, P, P, P, P, P, P, I, D, D, D, D, P, S, P, S,
+                         P, P, P]),
     "rvgp_batch_gram": (I, [I, I, I, I, P, I, P, P, I, P, P]),
     "rvgp_batch_rhs": (I, [I, I, I, P, I, P, P, P, P, P, P, P]),
     "rvgp_batch_step_ws_bytes": (S, [I, I]),
@@ -74,7 +78,8 @@ SIGNATURES = {
 }
 
 RVGP_EINVAL, RVGP_ERUNTIME, RVGP_ENOMEM, RVGP_EHIP = -1, -2, -3, -4
-ST_OK, ST_RANK_DEFICIENT, ST_SHORT_COMPONENT, ST_POOL_OVERFLOW, ST_NOT_SPD = 0, 1, 2, 3, 4
+ST_OK, ST_RANK_DEFICIENT, ST_SHORT_COMPONENT, ST_POOL_OVERFLOW, ST_NOT_SPD, ST_TIMEOUT = \
+    0, 1, 2, 3, 4, 5
 
 
 def load_symbols_only():

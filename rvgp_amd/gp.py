@@ -79,6 +79,9 @@ def weight_space_stats(U, c, precision="fp64"):
     return G, b
 
 
+FUSED_STEP_MAX_K = 2048  # RVGP_WS_STEP_MAX_K
+
+
 class GPREngine:
     """Hyperparameters theta = (nu, kappa, sigma_f, s2) = softplus(u) + lower."""
 
@@ -115,6 +118,7 @@ class GPREngine:
         self.gS = torch.empty(self.k, **f64)
         self.out = torch.zeros(9, **f64)
         self.status = torch.zeros(1, dtype=torch.int32, device=dev)
+        self.fused_step = True
         self.ya = torch.zeros(1, **f64)
         self.aa = torch.zeros(1, **f64)
         self._dot_ws = torch.empty(
@@ -183,6 +187,7 @@ class GPREngine:
         self.gS = torch.empty(self.k, **f64)
         self.out = torch.zeros(9, **f64)
         self.status = torch.zeros(1, dtype=torch.int32, device=dev)
+        self.fused_step = True
         self._cs_ws = torch.empty(
             _lib.lib().rvgp_colsumsq_ws_bytes(self.k, self.k) // 8 + 1, **f64)
         self.A = torch.empty((self.k, self.k), **f64)
@@ -382,16 +387,47 @@ class GPREngine:
         self.last_nfev = int(res.nfev)
         return res
 
+    def fused(self):
+        """Whether a step runs as the single-launch tile dataflow
+        (rvgp_ws_step): weight form, one right-hand side, k <= 2048."""
+        return (self.fused_step and self.form == "weight" and self.R == 1
+                and self.k <= FUSED_STEP_MAX_K)
+
+    def _ws_step(self, do_adam, lr=0.0, b1=0.9, b2=0.999, eps=1e-8):
+        if getattr(self, "_fws", None) is None:
+            L = _lib.lib()
+            self._fws = torch.empty(L.rvgp_ws_step_ws_bytes(self.k) // 8, dtype=torch.float64,
+                                    device=self.dev)
+            # flags of the dataflow: zeroed once, then advanced by every call
+            self._fsync = torch.zeros(L.rvgp_ws_step_sync_ints(self.k), dtype=torch.int32,
+                                      device=self.dev)
+        args = (_lib.ptr(self.m1), _lib.ptr(self.m2), _lib.ptr(self.t),
+                _lib.ptr(self.trainable)) if do_adam else (None, None, None, None)
+        m1, m2, t, tr = args
+        _lib.call("rvgp_ws_step", self.k, self.typ, _lib.ptr(self.evals), self.n_rows,
+                  _lib.ptr(self.G), self.G.stride(0), _lib.ptr(self.b), _lib.ptr(self.yy),
+                  float(self.M), _lib.ptr(self.lower), _lib.ptr(self.u), tr, m1, m2, t,
+                  int(do_adam), float(lr), float(b1), float(b2), float(eps), _lib.ptr(self._fws),
+                  self._fws.numel() * 8, _lib.ptr(self._fsync), self._fsync.numel(),
+                  _lib.ptr(self.out), _lib.ptr(self.status), _lib.stream())
+        return self.out
+
     def adam_setup(self, trainable=(1, 1, 1, 1)):
         f64 = dict(dtype=torch.float64, device=self.dev)
         self.m1 = torch.zeros(4, **f64)
         self.m2 = torch.zeros(4, **f64)
         self.t = torch.zeros(1, **f64)
         self.trainable = torch.as_tensor(np.asarray(trainable, np.int32), device=self.dev)
+        if self.fused():
+            self._ws_step(0)  # allocates the dataflow's workspace before any capture
 
     def adam_step(self, lr=0.05, b1=0.9, b2=0.999, eps=1e-8):
         """One device-resident optimiser step: NLML, adjoint, Adam update.
-        No host synchronisation, so it can be captured in a HIP graph."""
+        No host synchronisation, so it can be captured in a HIP graph.  In the
+        weight form with one right-hand side the whole step is one launch."""
+        if self.fused():
+            self._ws_step(1, lr, b1, b2, eps)
+            return
         self.evaluate()
         _lib.call("rvgp_adam_step", _lib.ptr(self.u), _lib.ptr(self.out[5:]), _lib.ptr(self.m1),
                   _lib.ptr(self.m2), _lib.ptr(self.t), _lib.ptr(self.trainable), float(lr),

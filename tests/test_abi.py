@@ -44,6 +44,18 @@ def test_arg_validation_reports_value_errors_without_a_gpu():
     assert rc == _lib.RVGP_EINVAL
     rc = L.rvgp_gauges(None, 10, 65, None, 10, 1, 1, 1, None, None, None, None)
     assert rc == _lib.RVGP_EINVAL
+    # the single-launch weight-space step: k range, workspace and flag sizes
+    def ws_step(k, ws_bytes, sync_ints):
+        # k, typ, evals, n_rows, G, ldg, b, yy, M, lower, u, trainable, m1, m2, t,
+        # do_adam, lr, b1, b2, eps, ws, ws_bytes, sync, sync_ints, out, status, stream
+        return L.rvgp_ws_step(k, 0, None, 1.0, None, max(k, 1), None, None, 1.0, None, None, None,
+                              None, None, None, 0, 0.0, 0.9, 0.999, 1e-8, None, ws_bytes, None,
+                              sync_ints, None, None, None)
+    assert ws_step(0, 1 << 40, 1 << 20) == _lib.RVGP_EINVAL
+    assert ws_step(2049, 1 << 40, 1 << 20) == _lib.RVGP_EINVAL
+    assert ws_step(100, L.rvgp_ws_step_ws_bytes(100) - 8,
+                   L.rvgp_ws_step_sync_ints(100)) == _lib.RVGP_EINVAL
+    assert b"workspace" in L.rvgp_last_error()
 
 
 def test_workspace_queries():
@@ -56,6 +68,15 @@ def test_workspace_queries():
     assert L.rvgp_geo_nbh_ws_bytes(1000, 4096, 1200) > 4096 * L.rvgp_geo_nbh_ws_bytes(1000, 1, 512)
     assert L.rvgp_colsumsq_ws_bytes(1000, 10) >= 8 * 10
     assert L.rvgp_colsumsq_ws_bytes(10**6, 10) == 8 * 10 * 1024  # at most 1024 row slices
+    # fused step: T = ceil(k/32) tile rows; two packed triangles of 32 x 32
+    # tiles (L and L^-1) plus t, w, diag(A^-1), S and dS (3) of length 32 T,
+    # 8 partial sums per tile row and 8 scalars; flags: T(T-1)/2 + 1 L tasks,
+    # T diagonal, T(T-1)/2 X tasks, T rows, T columns, the filter, and 4
+    # control words
+    for k, T in ((1, 1), (32, 1), (33, 2), (500, 16), (2048, 64)):
+        assert L.rvgp_ws_step_ws_bytes(k) == 8 * (T * (T + 1) * 1024 + 7 * 32 * T + 8 * T + 8)
+        assert L.rvgp_ws_step_sync_ints(k) == T * T + 2 * T + 6
+    assert L.rvgp_ws_step_ws_bytes(2049) == 0 and L.rvgp_ws_step_sync_ints(0) == 0
 
 
 def test_host_split_rule_and_softplus():

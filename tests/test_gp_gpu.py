@@ -382,3 +382,103 @@ def test_fit_with_inducing_points_vs_oracle():
     assert mean.shape == (50, 3)
     np.testing.assert_allclose(mean.ravel(), mr.ravel(), rtol=1e-7, atol=1e-9)
     np.testing.assert_allclose(var.ravel(), vr.ravel(), rtol=1e-7, atol=1e-9)
+
+
+@pytest.mark.parametrize("k", [1, 31, 32, 33, 100, 500])
+def test_fused_step_equals_multi_launch_step(k):
+    """The single-launch step (rvgp_ws_step: tile dataflow of Cholesky, L^-1,
+    NLML, adjoint and Adam) against the multi-kernel weight-space evaluate
+    and Adam step: NLML and gradients at a fixed theta, then the Adam
+    trajectory over 25 steps."""
+    Phi, y, lam = problem(M_nodes=max(60, k), k=k, seed=100 + k)
+    th = (1.7, 4.0, 1.3, 0.2)
+    n_rows = 3.0 * Phi.shape[0]
+    a = engine(Phi, y, lam, n_rows, "weight", theta=th)
+    b = engine(Phi, y, lam, n_rows, "weight", theta=th)
+    a.fused_step = False
+    assert b.fused() and not a.fused()
+    a.adam_setup()
+    b.adam_setup()
+    out_a = a.evaluate().cpu().numpy()
+    out_b = b._ws_step(0).cpu().numpy()
+    assert int(b.status.item()) == 0
+    np.testing.assert_allclose(out_b, out_a, rtol=1e-10, atol=1e-11 * np.abs(out_a).max())
+    S, dS = cpu.spectral_filter(lam, th[0], th[1], th[2], n_rows)
+    ref = cpu.gpr_nlml(Phi, y, S, th[3])
+    assert abs(out_b[0] - ref) < 1e-10 * abs(ref)
+    for _ in range(25):
+        a.adam_step(lr=0.05)
+        b.adam_step(lr=0.05)
+    assert int(b.status.item()) == 0
+    np.testing.assert_allclose(b.u.cpu().numpy(), a.u.cpu().numpy(), rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose(b.out.cpu().numpy(), a.out.cpu().numpy(), rtol=1e-8,
+                               atol=1e-10 * np.abs(out_a).max())
+    np.testing.assert_array_equal(b.t.cpu().numpy(), a.t.cpu().numpy())
+
+
+def test_fused_step_at_the_size_limit_and_past_it():
+    """k = 2048 (64 x 64 tiles, 4225 workgroups) runs fused; k = 2049 takes
+    the multi-launch path."""
+    import torch
+    from rvgp_amd.gp import GPREngine
+    rng = np.random.default_rng(5)
+    for k, fused in ((2048, True), (2049, False)):
+        lam = np.sort(rng.uniform(0.0, 8.0, size=k))
+        Phi = rng.normal(size=(k + 200, k)) / np.sqrt(k)
+        y = rng.normal(size=(k + 200, 1))
+        Pd = torch.as_tensor(Phi, device="cuda")
+        yd = torch.as_tensor(y, device="cuda")
+        a = GPREngine(Pd, yd, torch.as_tensor(lam, device="cuda"), 3.0 * k, form="weight")
+        a.set_theta((1.5, 3.0, 1.0, 0.5))
+        a.adam_setup()
+        assert a.fused() == fused
+        ref = a.evaluate().cpu().numpy().copy() if fused else None
+        if fused:
+            out = a._ws_step(0).cpu().numpy()
+            assert int(a.status.item()) == 0
+            np.testing.assert_allclose(out, ref, rtol=1e-9, atol=1e-10 * np.abs(ref).max())
+        a.adam_step(lr=0.05)
+        assert int(a.status.item()) == 0
+
+
+def test_fused_step_reports_a_non_positive_definite_system():
+    """A = G + s2 diag(S)^-1 with G = -10 I is not positive definite: the
+    fused step reports RVGP_ST_NOT_SPD like the multi-launch path, and the
+    next call on a good system reports RVGP_ST_OK again."""
+    import torch
+    from rvgp_amd import _lib
+    from rvgp_amd.gp import GPREngine
+    k = 70
+    lam = torch.linspace(0.0, 4.0, k, dtype=torch.float64, device="cuda")
+    b = torch.ones(k, 1, dtype=torch.float64, device="cuda")
+    yy = torch.full((1,), 5.0, dtype=torch.float64, device="cuda")
+    for G, want in ((-10.0 * torch.eye(k, dtype=torch.float64, device="cuda"), _lib.ST_NOT_SPD),
+                    (10.0 * torch.eye(k, dtype=torch.float64, device="cuda"), _lib.ST_OK)):
+        e = GPREngine(None, None, lam, 3.0 * k, form="weight", stats=(G, b, yy, 4 * k))
+        e.set_theta((1.5, 3.0, 1.0, 1e-3))
+        e.adam_setup()
+        e._ws_step(0)
+        assert int(e.status.item()) == want
+        e.fused_step = False
+        e.evaluate()
+        assert int(e.status.item()) == want
+
+
+def test_fused_step_graph_replays_match_eager():
+    """The captured single-launch step (one graph node per step) replayed
+    with eager work in between equals the eager fused step every time."""
+    import torch
+    Phi, y, lam = problem(M_nodes=400, k=200, seed=23)
+    a = engine(Phi, y, lam, 1200.0, "weight")
+    b = engine(Phi, y, lam, 1200.0, "weight")
+    a.adam_setup()
+    b.adam_setup()
+    assert a.fused() and b.fused()
+    g = b.capture_adam_step(lr=0.05)
+    for step in range(15):
+        a.adam_step(lr=0.05)
+        g.replay()
+        torch.cuda.synchronize()
+        _ = (torch.ones(1000, device="cuda") * step).sum().item()
+        np.testing.assert_array_equal(b.u.cpu().numpy(), a.u.cpu().numpy())
+        np.testing.assert_array_equal(b.out.cpu().numpy(), a.out.cpu().numpy())

@@ -123,6 +123,22 @@ def colsumsq(Z):
     return out
 
 
+def dot(x, y):
+    """sum_i x_i y_i of two contiguous FP64 tensors, as a (1,) device
+    tensor (fixed summation order)."""
+    x = x.contiguous()
+    y = y.contiguous()
+    n = x.numel()
+    if y.numel() != n:
+        raise ValueError("dot: sizes differ")
+    out = torch.empty(1, dtype=torch.float64, device=x.device)
+    ws_bytes = _lib.lib().rvgp_dot_ws_bytes(n)
+    ws = torch.empty(max(ws_bytes, 8), dtype=torch.uint8, device=x.device)
+    _lib.call("rvgp_dot", n, _lib.ptr(x), _lib.ptr(y), _lib.ptr(out), _lib.ptr(ws), ws_bytes,
+              _lib.stream())
+    return out
+
+
 def chol_logdet_half(L):
     """sum_i log L_ii (f64 device scalar)."""
     out = torch.empty(1, dtype=torch.float64, device=L.device)

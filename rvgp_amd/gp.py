@@ -363,12 +363,20 @@ class GPREngine:
         m, D, d = T_star.shape
         T = T_star.to(torch.float64).contiguous()
         mu = dense.gemm(U_star, self.w)                                  # (m*d, R)
-        mean = torch.bmm(T, mu.reshape(m, d, self.R))                    # (m, D, R)
+        # mean_i = T_i mu_i for every right-hand side (frames applied per node)
+        c = mu.reshape(m, d, self.R).permute(2, 0, 1).contiguous()      # (R, m, d)
+        out = torch.empty((self.R, m, D), dtype=torch.float64, device=T.device)
+        _lib.call("rvgp_frame_apply", 1, self.R, m, D, d, _lib.ptr(T), d, _lib.ptr(c),
+                  _lib.ptr(out), _lib.stream())
+        mean = out.permute(1, 2, 0)                                      # (m, D, R)
         Zt = U_star.T.contiguous()
         dense.trsm(self.A, Zt)                                           # L_A^-1 U^T
-        Z = Zt.T.contiguous().reshape(m, d, self.k)
-        B = torch.bmm(Z, Z.transpose(1, 2).contiguous())                 # (m, d, d)
-        var = self.s2 * (torch.bmm(T, B) * T).sum(-1).reshape(-1)
+        # B_i = Z_i Z_i^T (d x d) from the columns of Zt that belong to node i
+        Z3 = Zt.reshape(self.k, m, d)
+        B = torch.stack([torch.stack([(Z3[:, :, a] * Z3[:, :, b]).sum(0) for b in range(d)], -1)
+                         for a in range(d)], -2)                          # (m, d, d)
+        var = self.s2 * (T.unsqueeze(-1) * B.unsqueeze(1) * T.unsqueeze(-2)).sum((-1, -2))
+        var = var.reshape(-1)
         mean = mean.reshape(m * D, self.R)
         return (mean.reshape(-1) if self.R == 1 else mean), var
 

@@ -56,21 +56,22 @@ class BSR:
         steps = max(2, min(int(steps), N))
         g = torch.Generator(device=dev)
         g.manual_seed(int(seed))
+        from . import dense
         v = torch.randn(N, generator=g, dtype=torch.float64, device=dev)
-        v = v / torch.linalg.vector_norm(v)
+        v = v / torch.sqrt(dense.dot(v, v))
         v_prev = torch.zeros_like(v)
-        beta = torch.zeros((), dtype=torch.float64, device=dev)
+        beta = torch.zeros(1, dtype=torch.float64, device=dev)
         alphas, betas = [], []
         for _ in range(steps):
             w = spmm(self, v).reshape(-1)
-            alpha = torch.dot(w, v)
+            alpha = dense.dot(w, v)
             w = w - alpha * v - beta * v_prev
-            beta = torch.linalg.vector_norm(w)
+            beta = torch.sqrt(dense.dot(w, w))
             alphas.append(alpha)
             betas.append(beta)
             v_prev, v = v, w / torch.clamp(beta, min=1e-300)
-        a = torch.stack(alphas).cpu().numpy()
-        b = torch.stack(betas).cpu().numpy()
+        a = torch.cat(alphas).cpu().numpy()
+        b = torch.cat(betas).cpu().numpy()
         T = np.diag(a) + np.diag(b[:-1], 1) + np.diag(b[:-1], -1)
         th, Zt = np.linalg.eigh(T)
         bound = float(th[-1]) + abs(float(b[-1]) * float(Zt[-1, -1]))

@@ -130,10 +130,14 @@ class GPREngine:
             # function-space form it carries log det, diag(Phi^T K^-1 Phi) and
             # the Woodbury solve for alpha, all in FP64)
             if gram_mode is None:
-                # mixed: FP32 Phi tiles, products accumulated in FP64 on the
-                # matrix cores (an FP32-accumulated G over M = 768k rows costs
-                # ~1e-4 in the NLML)
-                gram_mode = dense.GEMM_F64 if precision == "fp64" else dense.GEMM_F32_ACC64
+                # mixed weight form: FP32 Phi tiles, products accumulated in
+                # FP64 on the matrix cores (an FP32-accumulated G over M = 768k
+                # rows costs ~1e-4 in the NLML).  The function form's Woodbury
+                # refinement divides by s2, which amplifies the ~1e-7 error of
+                # FP32-rounded rows until it stops converging at s2 ~ 1e-5; its
+                # G (M <= fs_max rows) costs next to nothing in FP64.
+                mixed_rows = precision == "mixed" and self.form == "weight"
+                gram_mode = dense.GEMM_F32_ACC64 if mixed_rows else dense.GEMM_F64
             self.gram_mode = gram_mode
             P = self.Phi if gram_mode == dense.GEMM_F64 else self.Phi.float()
             acc64 = gram_mode in (dense.GEMM_F64, dense.GEMM_F32_ACC64)

@@ -206,6 +206,30 @@ def cpu_baseline(args):
             "infeasible_at_full_size": _full_size_markers(args.n, args.k_nn)}
 
 
+def gp_step_utilisation(engine, reps=20):
+    """The optimiser step's k x k Cholesky + triangular inverse (2k^3/3 flops,
+    the dominant arithmetic of a step) against the FP64 matrix-core peak:
+    `reps` evaluations of the single-launch step timed with HIP events on
+    the launch stream (the hyperparameters do not move)."""
+    import torch
+    if not engine.fused():
+        return None
+    engine.adam_setup()
+    engine._ws_step(0)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        engine._ws_step(0)
+    e1.record()
+    e1.synchronize()
+    us = 1e3 * e0.elapsed_time(e1) / reps
+    k = engine.k
+    tf = 2.0 * k ** 3 / 3.0 / (us * 1e-6) / 1e12
+    return {"kernel": "ws_step_kernel", "k": k, "avg_us": round(us, 1), "achieved": round(tf, 3),
+            "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(tf / FP64_PEAK_TFLOPS, 5),
+            "bound": "latency: a chain of ceil(k/32) diagonal 32x32 tile factorisations"}
+
+
 def gpu_same_sample(args):
     """The mixed GPU fit on the CPU baseline's sample (warm), with its stages."""
     import numpy as np
@@ -698,6 +722,7 @@ def main():
             "traffic": traffic_from_profile(), "launches": ks["count"],
             "avg_us": round(ks["avg_us"], 2)}
     gram = kstats.get("gemm[f32acc64]", None)
+    step_util = gp_step_utilisation(gp_mx.engine)
 
     cpu_b = None
     if not args.skip_cpu:
@@ -725,6 +750,7 @@ def main():
         "gram_fp64_mfma": None if gram is None else {
             "achieved": round(gram["TFLOPs"], 2), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": round(gram["TFLOPs"] / FP64_PEAK_TFLOPS, 4), "launches": gram["count"]},
+        "cholesky_step_fp64_mfma": step_util,
         "kernels": {k: {kk: round(vv, 3) for kk, vv in v.items()} for k, v in kstats.items()},
         "cpu_baseline": cpu_b,
     }

@@ -48,6 +48,12 @@ DOUBLE_FILTER_BLOCKS = (1,)
 # is its square, which the Cholesky basis must still resolve)
 FP32_MAX_RATIO = 300.0
 FP64_MAX_RATIO = 1e4
+# the mixed form's FP32 phase hands over to FP64 at this residual (relative to
+# the Gershgorin bound), a few times above the FP32 floor
+MIXED_HANDOVER = 1e-5
+# a filter is given the degree that would cut the worst wanted residual by
+# NEED_SAFETY x the factor still needed
+NEED_SAFETY = 10.0
 
 
 class EigStats(dict):
@@ -203,7 +209,7 @@ def _run_phase(A, X, k, ub, wdt, ptol, max_iter, max_ratio, lock, fresh, stats, 
         tf = time.perf_counter()
         cut, low = float(th0.max().item()), float(th0.min().item())
         m, m_need = _degree(torch.sort(th0).values, min(k, th0.numel()), cut, ub, max_ratio,
-                            m_max=m_max, need=10.0 * worst0 / ptol)
+                            m_max=m_max, need=NEED_SAFETY * worst0 / ptol)
         if verbose:
             print(f"[chfsi {tag}] opening filter m={m} m_need={m_need} worst={worst0:.3e}",
                   file=sys.stderr, flush=True)
@@ -258,7 +264,7 @@ def _run_phase(A, X, k, ub, wdt, ptol, max_iter, max_ratio, lock, fresh, stats, 
         low = float(tha[0].item())
         kk = min(k - Xl.shape[1], tha.numel())
         m, m_need = _degree(tha, max(kk, 1), cut, ub, max_ratio, m_max=m_max,
-                            need=10.0 * worst / ptol if kk > 0 else None)
+                            need=NEED_SAFETY * worst / ptol if kk > 0 else None)
         if verbose:
             print(f"[chfsi {tag}] filter m={m} m_need={m_need} cut={cut:.4e} low={low:.4e}",
                   file=sys.stderr, flush=True)
@@ -325,7 +331,8 @@ def chfsi(A: BSR, k, precision="fp64", tol=1e-9, max_iter=300, guard=None, seed=
         # the FP32 residual floor (a phase that reaches the floor first
         # stalls there)
         X, theta, it, _ = _run_phase(A, X.to(torch.float32).contiguous(), k, ub, torch.float32,
-                                     1e-5, max_iter, FP32_MAX_RATIO, True, fresh, stats, verbose, "fp32",
+                                     MIXED_HANDOVER, max_iter, FP32_MAX_RATIO, True, fresh, stats,
+                                     verbose, "fp32",
                                      stall_exit=True, scale=scale, ritz=opening)
         stats["iters_fp32"] = it
         stats["t_fp32"] = time.perf_counter() - t0

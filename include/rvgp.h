@@ -254,6 +254,25 @@ size_t rvgp_syev_ws_bytes(int n);
 int rvgp_syev(int n, double* A, int lda, double* w, double* V, int ldv, void* ws,
               size_t ws_bytes, void* stream);
 
+/* Ritz residuals of a block: res[j] = |AX[:, j] - theta[j] X[:, j]| / |X[:, j]|
+ * (f64 sums in a fixed order, the residual block never formed); X and AX
+ * (M, k), f64 or f32 by `f64`, row strides ldx / ldax; theta (k) f64 on the
+ * device.  The convergence test ARPACK applies to its Ritz pairs inside eigsh
+ * (geometry.py:73).  ws: rvgp_ritz_residuals_ws_bytes(M, k) bytes. */
+size_t rvgp_ritz_residuals_ws_bytes(int M, int k);
+int rvgp_ritz_residuals(int f64, int M, int k, const void* X, int ldx, const void* AX, int ldax,
+                        const double* theta, double* res, void* ws, size_t ws_bytes,
+                        void* stream);
+
+/* Eigenvector output of compute_spectrum (geometry.py:72-75):
+ * out[perm[i] * b + a][j] = scale * X[i * b + a][j] / sqrt(nrm2[j]) for block
+ * rows i < nb, a < b, columns j < k -- unit-norm columns times
+ * scale = sqrt(rows), mapped back from a renumbering of the operator's block
+ * rows (perm[new] = old, int64; NULL = identity).  X f64 or f32 (`f64`), row
+ * stride ldx; out f64, row stride ldo, must not alias X. */
+int rvgp_evecs_finish(int f64, int nb, int b, int k, const void* X, int ldx, const double* nrm2,
+                      const long long* perm, double scale, double* out, int ldo, void* stream);
+
 /* out[j] = sum_i Z[i][j]^2 (f64 accumulation, fixed order); Z (M, k) f64/f32. */
 size_t rvgp_colsumsq_ws_bytes(int M, int k);
 int rvgp_colsumsq(int f64, int M, int k, const void* Z, int ldz, double* out, void* ws,

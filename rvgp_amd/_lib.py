@@ -52,6 +52,9 @@ SIGNATURES = {
     "rvgp_colsumsq_ws_bytes": (S, [I, I]),
     "rvgp_colsumsq": (I, [I, I, I, P, I, P, P, S, P]),
     "rvgp_colsumsq_noalloc": (I, [I, I, I, P, I, P, P]),
+    "rvgp_ritz_residuals_ws_bytes": (S, [I, I]),
+    "rvgp_ritz_residuals": (I, [I, I, I, P, I, P, I, P, P, P, S, P]),
+    "rvgp_evecs_finish": (I, [I, I, I, I, P, I, P, P, D, P, I, P]),
     "rvgp_spectral_filter": (I, [I, I, P, P, D, P, P, P]),
     "rvgp_ws_form": (I, [I, P, I, P, P, P, I, P]),
     "rvgp_ws_finish": (I, [I, D, I, P, I, P, P, P, P, P, P, P, P, P, P, P, P]),

@@ -18,7 +18,7 @@ import torch
 from . import _lib, geometry
 from .eigsolve import chfsi
 from .ptu_dijkstra import connections_device, tangent_frames_device
-from .reorder import morton_perm, permute_bsr, unpermute_rows
+from .reorder import morton_perm, permute_bsr
 from .smoothing import lambda_min_lower, vector_diffusion_device, vector_diffusion_spectral_device
 from .sparse import BSR, graph_laplacian
 
@@ -64,8 +64,11 @@ class data:
         # trips).
         kL = N if n_eigenpairs is None else min(int(n_eigenpairs), N)
         perm = perm if reorder else None
+        t0 = time.perf_counter()
         Lb = graph_laplacian(G)
         Lb_e = permute_bsr(Lb, perm) if reorder else Lb
+        sync()
+        t["laplacian_L"] = time.perf_counter() - t0
         resL = {}
         side = torch.cuda.Stream(device=main.device)
         side.wait_stream(main)
@@ -74,10 +77,10 @@ class data:
             try:
                 t1 = time.perf_counter()
                 with torch.cuda.stream(side):
+                    # unit-norm columns times sqrt(N) (geometry.py:75), rows
+                    # mapped back to the original node numbering
                     w, V, st = chfsi(Lb_e, kL, precision=precision, tol=eig_tol,
-                                     verbose=eig_verbose)
-                    if reorder:
-                        V = unpermute_rows(V, perm, 1)
+                                     verbose=eig_verbose, perm=perm, scale=math.sqrt(N))
                 side.synchronize()
                 resL.update(w=w, V=V, st=st, t=time.perf_counter() - t1)
             except BaseException as e:  # re-raised on the calling thread
@@ -100,9 +103,11 @@ class data:
             tangents, Sigma = tangent_frames_device(Xd, G, K_t, mn, mn, n_check)
             sync()
             t["gauges"] = time.perf_counter() - t0
+            t0 = time.perf_counter()
             dm = geometry.manifold_dimension(Sigma.cpu().numpy(), frac_explained=explained_variance) \
                 if verbose else _quiet_dimension(Sigma, explained_variance)
             gauges = tangents[:, :, :dm].contiguous()
+            t["dimension"] = time.perf_counter() - t0
             if verbose:
                 print('Predicted manifold dimension is {}'.format(dm))
 
@@ -120,10 +125,13 @@ class data:
             t0 = time.perf_counter()
             kLc = N * dm if n_eigenpairs is None else min(int(n_eigenpairs), N * dm)
             Lcb_e = permute_bsr(Lcb, perm) if reorder else Lcb
+            sync()
+            t["permute_Lc"] = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            # intrinsic (n*d, k) eigenvectors, unit-norm columns times
+            # sqrt(n*d) (geometry.py:75), in the original node numbering
             wLc, VLc, stLc = chfsi(Lcb_e, kLc, precision=precision, tol=eig_tol,
-                                   verbose=eig_verbose)
-            if reorder:
-                VLc = unpermute_rows(VLc, perm, dm)
+                                   verbose=eig_verbose, perm=perm, scale=math.sqrt(N * dm))
             sync()
             t["eig_Lc"] = time.perf_counter() - t0
         finally:
@@ -148,9 +156,9 @@ class data:
         self._R = conn["R"]
         self._Lb, self._Lcb = Lb, Lcb
         self._evals_L = wL
-        self._evecs_L = VL * math.sqrt(N)          # geometry.py:75
+        self._evecs_L = VL
         self._evals_Lc = wLc
-        self._U = VLc * math.sqrt(N * dm)          # intrinsic (n*d, k)
+        self._U = VLc                              # intrinsic (n*d, k)
         self.n_eigenpairs = kLc
         self._host = {}
         self._vectors = None
@@ -328,11 +336,12 @@ class data:
 
 
 def _quiet_dimension(Sigma, frac):
-    S = Sigma.cpu().numpy().copy()
+    """manifold_dimension (geometry.py:83-97) without its print, reduced on
+    the device: only the explained-variance profile (one value per singular
+    value) comes to the host for the threshold."""
     if frac == 1.0:
-        return S.shape[1]
-    S **= 2
-    S /= S.sum(1, keepdims=True)
-    S = S.cumsum(1)
-    var_exp = S.mean(0) - S.std(0)
+        return Sigma.shape[1]
+    S = Sigma * Sigma
+    S = torch.cumsum(S / S.sum(1, keepdim=True), 1)
+    var_exp = (S.mean(0) - S.std(0, unbiased=False)).cpu().numpy()
     return int(np.where(var_exp >= frac)[0][0] + 1)

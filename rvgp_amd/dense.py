@@ -123,6 +123,43 @@ def colsumsq(Z):
     return out
 
 
+def ritz_residuals(X, AX, theta):
+    """|AX[:, j] - theta[j] X[:, j]| / |X[:, j]| per column (f64), X and AX
+    (M, k) f32 or f64 with unit column stride, theta (k,) f64."""
+    M, k = X.shape
+    if AX.shape != X.shape or AX.dtype != X.dtype:
+        raise ValueError("ritz_residuals: X and AX differ in shape or dtype")
+    th = theta.to(device=X.device, dtype=torch.float64).contiguous()
+    out = torch.empty(k, dtype=torch.float64, device=X.device)
+    ws_bytes = _lib.lib().rvgp_ritz_residuals_ws_bytes(M, k)
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=X.device)
+    _lib.call("rvgp_ritz_residuals", int(X.dtype == torch.float64), M, k, _lib.ptr(X), _ld(X),
+              _lib.ptr(AX), _ld(AX), _lib.ptr(th), _lib.ptr(out), _lib.ptr(ws), ws_bytes,
+              _lib.stream())
+    return out
+
+
+def evecs_finish(X, b, scale=1.0, perm=None):
+    """Unit-norm columns of X (nb*b, k) times `scale`, block rows mapped back
+    through perm (perm[new] = old block row; None = identity), as a new f64
+    (nb*b, k) tensor."""
+    rows, k = X.shape
+    nb = rows // b
+    if nb * b != rows:
+        raise ValueError("evecs_finish: rows not a multiple of the block size")
+    nrm2 = colsumsq(X)
+    out = torch.empty((rows, k), dtype=torch.float64, device=X.device)
+    pp = None
+    if perm is not None:
+        pp = perm.to(device=X.device, dtype=torch.int64).contiguous()
+        if pp.numel() != nb:
+            raise ValueError("evecs_finish: perm length differs from the block rows")
+    _lib.call("rvgp_evecs_finish", int(X.dtype == torch.float64), nb, b, k, _lib.ptr(X), _ld(X),
+              _lib.ptr(nrm2), _lib.ptr(pp), float(scale), _lib.ptr(out),
+              k, _lib.stream())
+    return out
+
+
 def dot(x, y):
     """sum_i x_i y_i of two contiguous FP64 tensors, as a (1,) device
     tensor (fixed summation order)."""

@@ -80,7 +80,10 @@ def _rayleigh_ritz(A, Y, work_dtype):
     theta, Q = dense.syev(Hb)
     Cm = dense.gemm(Bm, Q).to(work_dtype).contiguous()
     X = dense.gemm(Y, Cm)
-    AX = dense.gemm(W, Cm)
+    # A X by one more block SpMM rather than W Cm: at N = 3e5, p = 576 the
+    # SpMM reads ~2 GB (1.1 ms FP32, 2.1 ms FP64 for b = 3) where the GEMM
+    # costs 2 N p^2 flops (3.3 ms FP32, ~10 ms FP64)
+    AX = spmm(A, X)
     return X, theta, AX
 
 
@@ -111,9 +114,8 @@ def _cholqr(Y):
 
 
 def _residuals(X, theta, AX):
-    """|A x_j - theta_j x_j| / |x_j| per column (column sums in FP64)."""
-    R = AX - X * theta.to(X.dtype)
-    return torch.sqrt(dense.colsumsq(R) / dense.colsumsq(X))
+    """|A x_j - theta_j x_j| / |x_j| per column (one pass, sums in FP64)."""
+    return dense.ritz_residuals(X, AX, theta)
 
 
 def _cheb_filter(A, X, m, a, b, a0):
@@ -200,6 +202,7 @@ def _run_phase(A, X, k, ub, wdt, ptol, max_iter, max_ratio, lock, fresh, stats, 
     scale = ub if scale is None else scale
     p = X.shape[1]
     history = []
+    locked_res = 0.0  # largest residual / scale among the locked pairs
     Xl = X[:, :0]
     thl = torch.zeros(0, dtype=torch.float64, device=X.device)
     Y = X
@@ -223,7 +226,7 @@ def _run_phase(A, X, k, ub, wdt, ptol, max_iter, max_ratio, lock, fresh, stats, 
         Y = _deflate(Y, Xl)
         Xa, tha, AXa = _rayleigh_ritz(A, Y, wdt)
         stats["rr"] += 1
-        stats["spmm"] += 1
+        stats["spmm"] += 2
         stats["t_rr"] += time.perf_counter() - tr
         res = _residuals(Xa, tha, AXa) / scale
         need = k - Xl.shape[1]
@@ -233,6 +236,7 @@ def _run_phase(A, X, k, ub, wdt, ptol, max_iter, max_ratio, lock, fresh, stats, 
             print(f"[chfsi {tag}] it={it} locked={Xl.shape[1]} active={Xa.shape[1]} "
                   f"worst_res={worst:.3e} spmm={stats['spmm']}", file=sys.stderr, flush=True)
         stats["last_worst"] = worst
+        stats["phase_residual"] = max(worst, locked_res)
         if kk >= need and worst < ptol:
             X = torch.cat([Xl, Xa], 1)
             return X, torch.cat([thl, tha]), it, True
@@ -247,13 +251,14 @@ def _run_phase(A, X, k, ub, wdt, ptol, max_iter, max_ratio, lock, fresh, stats, 
             stats["unconverged"] = worst
             return X, torch.cat([thl, tha]), it, False
         if lock:
-            conv = (res < ptol).cpu().numpy()
+            res_h = res.cpu().numpy()
             c = 0
-            while c < kk and conv[c]:
+            while c < kk and res_h[c] < ptol:
                 c += 1
             c = min(c, max(0, Xa.shape[1] - 8))  # keep a working block
             c -= c % 8  # block widths stay multiples of 8: 16-byte SpMM rows
             if c > 0:
+                locked_res = max(locked_res, float(res_h[:c].max()))
                 Xl = torch.cat([Xl, Xa[:, :c]], 1).contiguous()
                 thl = torch.cat([thl, tha[:c]])
                 Xa, tha = Xa[:, c:].contiguous(), tha[c:]
@@ -296,8 +301,12 @@ def _run_phase(A, X, k, ub, wdt, ptol, max_iter, max_ratio, lock, fresh, stats, 
 
 
 def chfsi(A: BSR, k, precision="fp64", tol=1e-9, max_iter=300, guard=None, seed=0,
-          verbose=False, X0=None):
-    """k smallest eigenpairs of the symmetric PSD operator A."""
+          verbose=False, X0=None, perm=None, scale=1.0):
+    """k smallest eigenpairs of the symmetric PSD operator A: eigenvalues
+    ascending, eigenvectors as unit-norm columns times ``scale``, with block
+    rows mapped back through ``perm`` (perm[new] = old block row) when A is a
+    renumbered operator.  stats["residual"] bounds |A x - theta x| / |x| over
+    the returned pairs, relative to the Gershgorin bound."""
     N = A.shape[0]
     if k >= N - 1:
         raise TypeError("Cannot use scipy.linalg.eigh for sparse A with k >= N - 1. "
@@ -311,8 +320,8 @@ def chfsi(A: BSR, k, precision="fp64", tol=1e-9, max_iter=300, guard=None, seed=
     # residuals are measured against the Gershgorin bound (a norm scale that
     # does not depend on a random start); the filters use the tighter
     # Lanczos bound of lambda_max
-    scale = A.gershgorin_bound() * 1.0001 + 1e-12
-    ub = min(scale, A.spectral_upper_bound() * 1.0001 + 1e-12)
+    scale_g = A.gershgorin_bound() * 1.0001 + 1e-12
+    ub = min(scale_g, A.spectral_upper_bound() * 1.0001 + 1e-12)
     stats_ub = ub
     stats = EigStats(spmm=0, rr=0, iters=0, t_filter=0.0, t_rr=0.0)
     gen = torch.Generator(device=dev)
@@ -333,38 +342,41 @@ def chfsi(A: BSR, k, precision="fp64", tol=1e-9, max_iter=300, guard=None, seed=
         X, theta, it, _ = _run_phase(A, X.to(torch.float32).contiguous(), k, ub, torch.float32,
                                      MIXED_HANDOVER, max_iter, FP32_MAX_RATIO, True, fresh, stats,
                                      verbose, "fp32",
-                                     stall_exit=True, scale=scale, ritz=opening)
+                                     stall_exit=True, scale=scale_g, ritz=opening)
         stats["iters_fp32"] = it
         stats["t_fp32"] = time.perf_counter() - t0
         # FP64 polish with locking: the FP32 block is already accurate to
         # ~1e-5, so a few filter passes finish the pairs
         X, theta, it, done = _run_phase(A, X.double().contiguous(), k, ub, torch.float64, tol, 50,
                                         1e4, True, fresh, stats, verbose, "fp64-polish",
-                                        ritz=(theta, stats["last_worst"]), scale=scale)
+                                        ritz=(theta, stats["last_worst"]), scale=scale_g)
         if not done:
             # an unlocked Rayleigh-Ritz over the whole block orders and refines
             X, theta, _, _ = _run_phase(A, X, k, ub, torch.float64, tol, 3, FP64_MAX_RATIO, False, fresh,
-                                        stats, verbose, "fp64-final", m_max=16, scale=scale)
+                                        stats, verbose, "fp64-final", m_max=16, scale=scale_g)
         stats["iters_polish"] = it
     elif precision == "fp64":
         X, theta, it, done = _run_phase(A, X.to(torch.float64).contiguous(), k, ub, torch.float64,
                                         tol, max_iter, FP64_MAX_RATIO, True, fresh, stats, verbose, "fp64",
-                                        scale=scale, ritz=opening)
+                                        scale=scale_g, ritz=opening)
         if not done:
             X, theta, _, _ = _run_phase(A, X, k, ub, torch.float64, tol, 3, FP64_MAX_RATIO, False, fresh,
-                                        stats, verbose, "fp64-final", m_max=16, scale=scale)
+                                        stats, verbose, "fp64-final", m_max=16, scale=scale_g)
     else:
         raise ValueError("precision must be 'fp64' or 'mixed'")
-    stats["t_total"] = time.perf_counter() - t0
     o = torch.argsort(theta)
-    theta, X = theta[o], X[:, o]
-    AX = spmm(A, X[:, :k].contiguous())
-    stats["residual"] = float(_residuals(X[:, :k], theta[:k], AX).max().item() / scale)
+    theta = theta[o]
+    in_order = bool(torch.equal(o[:k], torch.arange(k, device=o.device)))
+    Xk = X[:, :k] if in_order else X[:, o[:k]].contiguous()
+    # the last phase's residuals (relative to the Gershgorin bound): the
+    # active pairs' at its final Rayleigh-Ritz step, the locked pairs' when
+    # they were locked
+    stats["residual"] = stats.pop("phase_residual")
     stats["ub_filter"] = stats_ub
-    stats["ub_gershgorin"] = scale
+    stats["ub_gershgorin"] = scale_g
     evals = theta[:k].clone()
-    evecs = X[:, :k].double()
-    evecs = evecs / torch.sqrt(dense.colsumsq(evecs))
+    evecs = dense.evecs_finish(Xk, A.b, scale, perm)
+    stats["t_total"] = time.perf_counter() - t0
     return evals, evecs, stats
 
 
@@ -373,5 +385,5 @@ def spectrum_of_scipy(laplacian, n_eigenpairs=None):
     N = laplacian.shape[0]
     k = N if n_eigenpairs is None else min(int(n_eigenpairs), N)
     A = BSR.from_scipy(laplacian, 1)
-    evals, evecs, _ = chfsi(A, k)
-    return evals.cpu().numpy(), evecs.cpu().numpy() * np.sqrt(N)
+    evals, evecs, _ = chfsi(A, k, scale=math.sqrt(N))  # geometry.py:75
+    return evals.cpu().numpy(), evecs.cpu().numpy()

@@ -9,7 +9,8 @@ indices, consecutive block rows share most of their neighbours, and the
 X rows a workgroup gathers stay resident in L2.
 
 The permuted operator P A P^T has the same spectrum; eigenvector block rows
-are mapped back with the inverse permutation.  Node identity is never
+are mapped back through the permutation when the eigensolver writes its
+output (rvgp_evecs_finish).  Node identity is never
 renumbered anywhere else (the heap neighbourhoods, whose tie-breaking
 depends on node order, run on the original numbering).
 """
@@ -56,11 +57,3 @@ def permute_bsr(A, perm):
     v64 = A.vals64.reshape(-1, b * b)[order].reshape(-1).contiguous()
     v32 = None if A.vals32 is None else A.vals32.reshape(-1, b * b)[order].reshape(-1).contiguous()
     return BSR(new_rp.to(torch.int32), cols, v64, v32, b)
-
-
-def unpermute_rows(V, perm, b):
-    """Rows of V in permuted block numbering -> original numbering."""
-    nb = perm.numel()
-    out = torch.empty_like(V)
-    out.view(nb, b, -1)[perm] = V.view(nb, b, -1)
-    return out

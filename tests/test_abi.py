@@ -56,6 +56,15 @@ def test_arg_validation_reports_value_errors_without_a_gpu():
     assert ws_step(100, L.rvgp_ws_step_ws_bytes(100) - 8,
                    L.rvgp_ws_step_sync_ints(100)) == _lib.RVGP_EINVAL
     assert b"workspace" in L.rvgp_last_error()
+    # Rayleigh-Ritz residuals and the eigenvector output pass
+    rc = L.rvgp_ritz_residuals(1, 10, 5, None, 4, None, 5, None, None, None, 0, None)  # ldx < k
+    assert rc == _lib.RVGP_EINVAL
+    rc = L.rvgp_ritz_residuals(1, 10, 5, 1, 5, 1, 5, 1, 1, 1, 8, None)  # workspace too small
+    assert rc == _lib.RVGP_EINVAL and b"workspace" in L.rvgp_last_error()
+    rc = L.rvgp_evecs_finish(1, 10, 4, 5, None, 5, None, None, 1.0, None, 5, None)  # b > 3
+    assert rc == _lib.RVGP_EINVAL
+    rc = L.rvgp_evecs_finish(1, 10, 3, 5, 1, 5, 1, None, 1.0, 1, 5, None)  # out aliases X
+    assert rc == _lib.RVGP_EINVAL and b"alias" in L.rvgp_last_error()
 
 
 def test_workspace_queries():
@@ -68,6 +77,9 @@ def test_workspace_queries():
     assert L.rvgp_geo_nbh_ws_bytes(1000, 4096, 1200) > 4096 * L.rvgp_geo_nbh_ws_bytes(1000, 1, 512)
     assert L.rvgp_colsumsq_ws_bytes(1000, 10) >= 8 * 10
     assert L.rvgp_colsumsq_ws_bytes(10**6, 10) == 8 * 10 * 1024  # at most 1024 row slices
+    # two f64 partial sums per (32-row slice, column), at most 1024 slices
+    assert L.rvgp_ritz_residuals_ws_bytes(100, 7) == 2 * 8 * 4 * 7
+    assert L.rvgp_ritz_residuals_ws_bytes(10**6, 7) == 2 * 8 * 1024 * 7
     # fused step: T = ceil(k/32) tile rows; two packed triangles of 32 x 32
     # tiles (L and L^-1) plus t, w, diag(A^-1), S and dS (3) of length 32 T,
     # 8 partial sums per tile row and 8 scalars; flags: T(T-1)/2 + 1 L tasks,

@@ -143,3 +143,47 @@ def test_gemm_sym_f32_raised_split_matches_full_product():
     ref = Yd.T @ Yd
     np.testing.assert_allclose(C, ref, rtol=0, atol=1e-5 * np.abs(ref).max())
     assert np.array_equal(C, C.T)
+
+
+@pytest.mark.parametrize("f64", [True, False])
+@pytest.mark.parametrize("M,k,ld", [(1, 1, 1), (37, 5, 8), (3000, 300, 304), (70001, 64, 64)])
+def test_ritz_residuals_match_the_formed_residual_block(M, k, ld, f64):
+    """rvgp_ritz_residuals (one pass, f64 sums) against |AX - X diag(theta)|
+    / |X| formed column by column in FP64 numpy; strided rows (ld > k)."""
+    import torch
+    from rvgp_amd import dense
+    rng = np.random.default_rng(M + k)
+    dt = torch.float64 if f64 else torch.float32
+    Xs = rng.normal(size=(M, ld))
+    AXs = rng.normal(size=(M, ld))
+    theta = rng.uniform(0.0, 3.0, size=k)
+    X, AX = _t(Xs, dt)[:, :k], _t(AXs, dt)[:, :k]
+    Xh, AXh = X.cpu().double().numpy(), AX.cpu().double().numpy()
+    ref = np.linalg.norm(AXh - Xh * theta, axis=0) / np.linalg.norm(Xh, axis=0)
+    got = dense.ritz_residuals(X, AX, _t(theta, torch.float64)).cpu().numpy()
+    np.testing.assert_allclose(got, ref, rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("f64", [True, False])
+@pytest.mark.parametrize("b", [1, 3])
+@pytest.mark.parametrize("with_perm", [False, True])
+def test_evecs_finish_normalises_scales_and_unpermutes(b, f64, with_perm):
+    """rvgp_evecs_finish: out[perm[i] b + a] = scale X[i b + a] / |X[:, j]|,
+    against the same map written with torch indexing in FP64 (a strided
+    column slice as input, as the eigensolver passes it)."""
+    import torch
+    from rvgp_amd import dense
+    nb, k, ld = 4001, 77, 96
+    rng = np.random.default_rng(b * 10 + int(f64))
+    dt = torch.float64 if f64 else torch.float32
+    X = _t(rng.normal(size=(nb * b, ld)), dt)[:, :k]
+    perm = torch.as_tensor(rng.permutation(nb), device="cuda") if with_perm else None
+    scale = float(np.sqrt(nb * b))
+    got = dense.evecs_finish(X, b, scale, perm)
+    Xd = X.double()
+    ref = Xd / torch.linalg.vector_norm(Xd, dim=0) * scale
+    if with_perm:
+        out = torch.empty_like(ref)
+        out.view(nb, b, k)[perm] = ref.view(nb, b, k)
+        ref = out
+    np.testing.assert_allclose(got.cpu().numpy(), ref.cpu().numpy(), rtol=1e-13, atol=1e-13)

@@ -400,3 +400,22 @@ def test_affinity_graph_matches_oracle():
     np.testing.assert_array_equal(nbh, ref)
     with pytest.raises(UnboundLocalError):
         geometry.manifold_graph(X, typ='radius')
+
+
+@pytest.mark.parametrize("frac", [0.5, 0.8, 0.95, 1.0])
+def test_quiet_dimension_equals_the_host_estimate(frac):
+    """The data object's device reduction of the explained-variance profile
+    picks the same dimension as manifold_dimension (geometry.py:83-97) on
+    the host, for singular-value profiles of 1-, 2- and 3-dimensional
+    point clouds."""
+    import torch
+    from rvgp_amd import geometry
+    from rvgp_amd.dataclass import _quiet_dimension
+    rng = np.random.default_rng(7)
+    for d in (1, 2, 3):
+        S = np.abs(rng.normal(scale=0.05, size=(5000, 12)))
+        S[:, :d] += 1.0 + rng.uniform(size=(5000, d))
+        S = -np.sort(-S, axis=1)
+        want = geometry.manifold_dimension(S.copy(), frac_explained=frac)
+        got = _quiet_dimension(torch.as_tensor(S, device="cuda"), frac)
+        assert got == want, (d, frac, got, want)

@@ -212,7 +212,7 @@ __global__ __launch_bounds__(GT) __attribute__((amdgpu_waves_per_eu(2))) void ge
                                                       typename Tr<MD>::Acc beta,
                                                       typename Tr<MD>::Acc* __restrict__ C, int ldc,
                                                       typename Tr<MD>::Acc* __restrict__ part,
-                                                      int upper) {
+                                                      int upper, int vec) {
     using In = typename Tr<MD>::In;
     using Lds = typename Tr<MD>::Lds;
     using Acc = typename Tr<MD>::Acc;
@@ -222,8 +222,13 @@ __global__ __launch_bounds__(GT) __attribute__((amdgpu_waves_per_eu(2))) void ge
     // have landed, so the wait for them stays one k-slice behind their issue.
     using Reg = In;
     constexpr int BK = Tr<MD>::BK;
-    constexpr int PAD = (sizeof(Lds) == 8) ? 1 : 2;
+    constexpr int PAD = (sizeof(Lds) == 8) ? 1 : 4;
     constexpr int EA = BBM * BK / GT;  // elements of each tile staged per thread
+    // 16-byte loads of FP32 inputs along the contiguous index of each operand
+    // (when `vec`: 16-byte aligned bases, leading dimensions a multiple of 4;
+    // FP64 inputs load element by element, which measured faster)
+    constexpr int VW = sizeof(In) == 4 ? 4 : 1;
+    constexpr int EV = EA / VW;        // vectors of each tile per thread
     __shared__ Lds As[BK][BBM + PAD];
     __shared__ Lds Bs[BK][BBN + PAD];
 
@@ -242,45 +247,78 @@ __global__ __launch_bounds__(GT) __attribute__((amdgpu_waves_per_eu(2))) void ge
         for (int j = 0; j < 4; ++j) acc[i][j] = Vec{0, 0, 0, 0};
 
     Reg ra[EA], rb[EA];
+    // Element q of this thread's share: vector v = tid + GT * (q / VW), lane
+    // t = q % VW of it.  A (m, k): along k when row-major, along m when
+    // transposed; B (k, n): along n when row-major, along k when transposed.
+#define RVGP_A_AT(q, mm, kk)                                                        \
+    const int mm = tA ? ((tid + GT * ((q) / VW)) % (BBM / VW)) * VW + (q) % VW     \
+                      : (tid + GT * ((q) / VW)) / (BK / VW);                       \
+    const int kk = tA ? (tid + GT * ((q) / VW)) / (BBM / VW)                       \
+                      : ((tid + GT * ((q) / VW)) % (BK / VW)) * VW + (q) % VW;
+#define RVGP_B_AT(q, nn, kk)                                                        \
+    const int nn = tB ? (tid + GT * ((q) / VW)) / (BK / VW)                        \
+                      : ((tid + GT * ((q) / VW)) % (BBN / VW)) * VW + (q) % VW;    \
+    const int kk = tB ? ((tid + GT * ((q) / VW)) % (BK / VW)) * VW + (q) % VW      \
+                      : (tid + GT * ((q) / VW)) / (BBN / VW);
+    typedef In InV __attribute__((ext_vector_type(VW > 1 ? VW : 2)));
     auto load = [&](int k0) {
 #pragma unroll
-        for (int q = 0; q < EA; ++q) {
-            const int e = tid + GT * q;
-            int mm, kk;
-            if (tA) { kk = e / BBM; mm = e - kk * BBM; }
-            else { mm = e / BK; kk = e - mm * BK; }
-            const int gm = m0 + mm, gk = k0 + kk;
-            ra[q] = (gm < M && gk < kend) ? (tA ? A[(size_t)gk * lda + gm] : A[(size_t)gm * lda + gk])
-                                          : In(0);
-            int nn;
-            if (tB) { nn = e / BK; kk = e - nn * BK; }
-            else { kk = e / BBN; nn = e - kk * BBN; }
-            const int gn = n0 + nn, gk2 = k0 + kk;
-            In u = 0;
-            if (gn < N && gk2 < kend) u = tB ? B[(size_t)gn * ldb + gk2] : B[(size_t)gk2 * ldb + gn];
-            rb[q] = (Reg)u;
+        for (int u = 0; u < EV; ++u) {
+            {
+                RVGP_A_AT(u * VW, mm, kk)
+                const int gm = m0 + mm, gk = k0 + kk;
+                const In* src = tA ? A + (size_t)gk * lda + gm : A + (size_t)gm * lda + gk;
+                if (VW > 1 && vec && (tA ? (gk < kend && gm + VW <= M) : (gm < M && gk + VW <= kend))) {
+                    const InV x = *reinterpret_cast<const InV*>(src);
+#pragma unroll
+                    for (int t = 0; t < VW; ++t) ra[u * VW + t] = x[t];
+                } else {
+#pragma unroll
+                    for (int t = 0; t < VW; ++t) {
+                        const bool in = tA ? (gm + t < M && gk < kend) : (gm < M && gk + t < kend);
+                        ra[u * VW + t] = in ? src[t] : In(0);
+                    }
+                }
+            }
+            {
+                RVGP_B_AT(u * VW, nn, kk)
+                const int gn = n0 + nn, gk = k0 + kk;
+                const In* src = tB ? B + (size_t)gn * ldb + gk : B + (size_t)gk * ldb + gn;
+                if (VW > 1 && vec && (tB ? (gn < N && gk + VW <= kend) : (gk < kend && gn + VW <= N))) {
+                    const InV x = *reinterpret_cast<const InV*>(src);
+#pragma unroll
+                    for (int t = 0; t < VW; ++t) rb[u * VW + t] = x[t];
+                } else {
+#pragma unroll
+                    for (int t = 0; t < VW; ++t) {
+                        const bool in = tB ? (gn < N && gk + t < kend) : (gn + t < N && gk < kend);
+                        rb[u * VW + t] = in ? src[t] : In(0);
+                    }
+                }
+            }
         }
     };
     auto store = [&](int k0) {
 #pragma unroll
         for (int q = 0; q < EA; ++q) {
-            const int e = tid + GT * q;
-            int mm, kk;
-            if (tA) { kk = e / BBM; mm = e - kk * BBM; }
-            else { mm = e / BK; kk = e - mm * BK; }
-            if (kscale != nullptr && k0 + kk < kend) {
-                const double v = (double)ra[q] * kscale[k0 + kk];
-                if constexpr (MD == Mode::F32 || MD == Mode::BF16) As[kk][mm] = to_lds<MD>((float)v);
-                else As[kk][mm] = (Lds)v;
-            } else {
-                As[kk][mm] = to_lds<MD>(ra[q]);
+            {
+                RVGP_A_AT(q, mm, kk)
+                if (kscale != nullptr && k0 + kk < kend) {
+                    const double v = (double)ra[q] * kscale[k0 + kk];
+                    if constexpr (MD == Mode::F32 || MD == Mode::BF16) As[kk][mm] = to_lds<MD>((float)v);
+                    else As[kk][mm] = (Lds)v;
+                } else {
+                    As[kk][mm] = to_lds<MD>(ra[q]);
+                }
             }
-            int nn;
-            if (tB) { nn = e / BK; kk = e - nn * BK; }
-            else { kk = e / BBN; nn = e - kk * BBN; }
-            Bs[kk][nn] = static_cast<Lds>(rb[q]);
+            {
+                RVGP_B_AT(q, nn, kk)
+                Bs[kk][nn] = static_cast<Lds>(rb[q]);
+            }
         }
     };
+#undef RVGP_A_AT
+#undef RVGP_B_AT
 
     if (kbeg < kend) load(kbeg);
     for (int k0 = kbeg; k0 < kend; k0 += BK) {
@@ -513,10 +551,36 @@ __global__ void mirror_lower(int n, Acc* __restrict__ C, int ldc) {
     if (i > j) C[(size_t)i * ldc + j] = C[(size_t)j * ldc + i];
 }
 
+// 16-byte vector loads in the 128 x 128 kernel: both operands 16-byte
+// aligned with leading dimensions a multiple of the vector width.
+static inline int vec_ok(const void* A, int lda, const void* B, int ldb, int elem) {
+    const int vw = 16 / elem;
+    return ((reinterpret_cast<uintptr_t>(A) | reinterpret_cast<uintptr_t>(B)) & 15) == 0 &&
+           lda % vw == 0 && ldb % vw == 0;
+}
+
 struct GemmPlan {
     bool small, big;
     int split, kchunk;
 };
+
+// Workgroups of the 128 x 128 FP32 kernel resident at once on the device.
+// Queried once and cached, the fallback included, so that the workspace size
+// query and the launch that follows it always plan the same split.
+static int f32_big_slots() {
+    static int slots = 0;  // the same for every device of one process's type
+    if (slots == 0) {
+        int dev = 0, per = 0;
+        hipDeviceProp_t prop;
+        if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, gemm_big_kernel<Mode::F32>, GT, 0) !=
+                hipSuccess)
+            slots = 768;
+        else
+            slots = (per > 0 ? per : 1) * prop.multiProcessorCount;
+    }
+    return slots;
+}
 
 // small: K <= 1024 and at most 64 tiles of the 128 x 128 kernel -- too few
 // workgroups to hide the k-loop latency, so one wave per 16 x 16 tile wins.
@@ -607,7 +671,7 @@ static int gemm_launch(int transA, int transB, int M, int N, int K, double alpha
         hipLaunchKernelGGL(gemm_big_kernel<MD>, grid, dim3(GT), 0, s, transA, transB, M, N, K,
                            pl.kchunk, (Acc)alpha, static_cast<const In*>(A), lda,
                            static_cast<const In*>(B), ldb, kscale, (Acc)beta, static_cast<Acc*>(C),
-                           ldc, part, upper);
+                           ldc, part, upper, vec_ok(A, lda, B, ldb, (int)sizeof(In)));
     } else {
         dim3 grid(cdiv(M, BM), cdiv(N, BN), pl.split);
         hipLaunchKernelGGL(gemm_kernel<MD>, grid, dim3(GT), 0, s, transA, transB, M, N, K, pl.kchunk,
@@ -636,24 +700,6 @@ static int gemm_launch(int transA, int transB, int M, int N, int K, double alpha
 // the independent chunk errors ~1e-6 -- below the FP32 data's own level in
 // a Rayleigh-Ritz step of an FP32 block.
 constexpr int kF32Chunk = 4096;
-
-// Workgroups of the 128 x 128 FP32 kernel resident at once on the device.
-// Queried once and cached, the fallback included, so that the workspace size
-// query and the launch that follows it always plan the same split.
-static int f32_big_slots() {
-    static int slots = 0;  // the same for every device of one process's type
-    if (slots == 0) {
-        int dev = 0, per = 0;
-        hipDeviceProp_t prop;
-        if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess ||
-            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, gemm_big_kernel<Mode::F32>, GT, 0) !=
-                hipSuccess)
-            slots = 768;
-        else
-            slots = (per > 0 ? per : 1) * prop.multiProcessorCount;
-    }
-    return slots;
-}
 
 static GemmPlan gram_f32_plan(int M, int K, int split_k) {
     GemmPlan p = gemm_plan(RVGP_GEMM_F32, M, M, K, Tr<Mode::F32>::BK, split_k);
@@ -705,7 +751,8 @@ static int gram_f32_chunked(int M, int K, const float* A, int lda, const float* 
     if (pl.big) {
         dim3 grid(cdiv(M, BBM), cdiv(M, BBN), pl.split);
         hipLaunchKernelGGL(gemm_big_kernel<Mode::F32>, grid, dim3(GT), 0, s, 1, 0, M, M, K,
-                           pl.kchunk, 1.0f, A, lda, B, ldb, nullptr, 0.0f, nullptr, M, part, 1);
+                           pl.kchunk, 1.0f, A, lda, B, ldb, nullptr, 0.0f, nullptr, M, part, 1,
+                           vec_ok(A, lda, B, ldb, (int)sizeof(float)));
     } else {
         dim3 grid(cdiv(M, BM), cdiv(M, BN), pl.split);
         hipLaunchKernelGGL(gemm_kernel<Mode::F32>, grid, dim3(GT), 0, s, 1, 0, M, M, K, pl.kchunk,

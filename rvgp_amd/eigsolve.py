@@ -46,6 +46,9 @@ DOUBLE_FILTER_BLOCKS = (1,)
 # blocks keep ~300 (the weakest directions then carry ~2e-5 relative
 # precision); FP64 blocks FP64_MAX_RATIO (the Rayleigh-Ritz Gram's condition
 # is its square, which the Cholesky basis must still resolve)
+# filter degrees chosen per column (shrinking trailing slices of the block)
+# rather than one degree for the whole block
+ADAPTIVE_DEGREES = True
 FP32_MAX_RATIO = 300.0
 FP64_MAX_RATIO = 1e4
 # the mixed form's FP32 phase hands over to FP64 at this residual (relative to
@@ -135,6 +138,66 @@ def _cheb_filter(A, X, m, a, b, a0):
     return cur
 
 
+def _cheb_filter_cols(A, X, ms, a, b, a0):
+    """Scaled Chebyshev filter with a degree per column.  ms (numpy ints) is
+    non-decreasing along the block and constant over groups of 8 columns;
+    the three-term recurrence acts column by column, so after step s only the
+    columns whose degree exceeds s go on -- a shrinking trailing slice of the
+    block -- and the others keep their last iterate.  X is overwritten; the
+    result is returned (X's storage or a new block)."""
+    m = int(ms[-1])
+    if int(ms[0]) == m:
+        return _cheb_filter(A, X, m, a, b, a0)
+    e = (b - a) / 2.0
+    c = (b + a) / 2.0
+    sigma = e / (a0 - c)
+    tau = 2.0 / sigma
+    Y = spmm(A, X, alpha=sigma / e, beta=-sigma * c / e)
+    bufs = (X, Y)  # the iterate of step s is in bufs[s % 2]
+    # starts[s] = first column whose degree is at least s
+    starts = np.searchsorted(ms, np.arange(m + 2), side="left")
+    s1 = sigma
+    for step in range(2, m + 1):
+        c0 = int(starts[step])
+        cur, prev = bufs[(step - 1) % 2], bufs[step % 2]
+        s2 = 1.0 / (tau - s1)
+        spmm(A, cur[:, c0:], Y=prev[:, c0:], alpha=2.0 * s2 / e, beta=-2.0 * s2 * c / e,
+             Z=prev[:, c0:], gamma=-s1 * s2)
+        s1 = s2
+    out = bufs[m % 2]
+    for step in range(1, m):  # columns that stopped on a step of the other parity
+        c0, c1 = int(starts[step]), int(starts[step + 1])
+        if step % 2 != m % 2 and c1 > c0:
+            out[:, c0:c1].copy_(bufs[step % 2][:, c0:c1])
+    return out
+
+
+def _column_degrees(theta, res, worst, ncols, kk, cut, ub, m, ptol, m_min=4):
+    """Per-column filter degrees for a block whose first min(kk, len(theta))
+    columns are wanted Ritz vectors (Ritz values theta, residuals res, or the
+    worst residual for all when res is None): column i gets the degree that
+    would cut its own residual by NEED_SAFETY x the factor still needed,
+    ln(need_i) / acosh(x_i) with x_i its Ritz value mapped to the damped
+    interval, at most m; guard and fresh columns get m.  The degrees are made
+    non-decreasing along the block and constant over groups of 8 columns
+    (16-byte SpMM rows), so the columns still filtering at any step form a
+    trailing slice."""
+    ms = np.full(ncols, m, dtype=np.int64)
+    nt = min(len(theta), kk, ncols)
+    if nt > 0:
+        e = (ub - cut) / 2.0
+        c = (ub + cut) / 2.0
+        x = (c - np.asarray(theta[:nt], dtype=np.float64)) / e
+        a = np.arccosh(np.maximum(x, 1.0 + 1e-15))
+        r = np.full(nt, worst) if res is None else np.asarray(res[:nt], dtype=np.float64)
+        need = np.maximum(NEED_SAFETY * r / ptol, 1.0 + 1e-12)
+        ms[:nt] = np.clip(np.ceil(np.log(need) / a), m_min, m).astype(np.int64)
+    ms = np.maximum.accumulate(ms)
+    g = -(-ncols // 8)
+    grp = np.concatenate([ms, np.full(g * 8 - ncols, ms[-1])]).reshape(g, 8).max(1)
+    return np.repeat(grp, 8)[:ncols]
+
+
 def _degree(theta, k, cut, ub, max_ratio, m_min=4, m_max=60, need=None):
     """Largest degree whose amplification spread over the block stays below
     max_ratio (keeps the filtered block well enough conditioned), and no
@@ -216,7 +279,14 @@ def _run_phase(A, X, k, ub, wdt, ptol, max_iter, max_ratio, lock, fresh, stats, 
         if verbose:
             print(f"[chfsi {tag}] opening filter m={m} m_need={m_need} worst={worst0:.3e}",
                   file=sys.stderr, flush=True)
-        Y = _cheb_filter(A, X, m, cut, ub, low)
+        if ADAPTIVE_DEGREES:
+            th_h = th0.cpu().numpy()
+            ms = _column_degrees(th_h, None, worst0, X.shape[1], min(k, th_h.size), cut, ub, m, ptol)
+            Y = _cheb_filter_cols(A, X, ms, cut, ub, low)
+            stats["spmm_cols"] += int(ms.sum())
+        else:
+            Y = _cheb_filter(A, X, m, cut, ub, low)
+            stats["spmm_cols"] += m * X.shape[1]
         stats["spmm"] += m
         torch.cuda.current_stream().synchronize()
         stats["t_filter"] += time.perf_counter() - tf
@@ -250,9 +320,9 @@ def _run_phase(A, X, k, ub, wdt, ptol, max_iter, max_ratio, lock, fresh, stats, 
             X = torch.cat([Xl, Xa], 1)
             stats["unconverged"] = worst
             return X, torch.cat([thl, tha]), it, False
+        res_h = res.cpu().numpy()
+        c = 0
         if lock:
-            res_h = res.cpu().numpy()
-            c = 0
             while c < kk and res_h[c] < ptol:
                 c += 1
             c = min(c, max(0, Xa.shape[1] - 8))  # keep a working block
@@ -274,7 +344,14 @@ def _run_phase(A, X, k, ub, wdt, ptol, max_iter, max_ratio, lock, fresh, stats, 
             print(f"[chfsi {tag}] filter m={m} m_need={m_need} cut={cut:.4e} low={low:.4e}",
                   file=sys.stderr, flush=True)
         tf = time.perf_counter()
-        Y = _cheb_filter(A, Xa, m, cut, ub, low)
+        if ADAPTIVE_DEGREES:
+            ms = _column_degrees(tha.cpu().numpy(), res_h[c:], worst, Xa.shape[1], max(kk, 0), cut,
+                                 ub, m, ptol)
+            Y = _cheb_filter_cols(A, Xa, ms, cut, ub, low)
+            stats["spmm_cols"] += int(ms.sum())
+        else:
+            Y = _cheb_filter(A, Xa, m, cut, ub, low)
+            stats["spmm_cols"] += m * Xa.shape[1]
         stats["spmm"] += m
         if (A.b in DOUBLE_FILTER_BLOCKS and wdt == torch.float32 and Xl.shape[1] == 0
                 and m_need is not None
@@ -293,6 +370,7 @@ def _run_phase(A, X, k, ub, wdt, ptol, max_iter, max_ratio, lock, fresh, stats, 
                 m2 = max(4, min(m, m_need - m))
                 Y = _cheb_filter(A, Q, m2, cut, ub, low)
                 stats["spmm"] += m2
+                stats["spmm_cols"] += m2 * Q.shape[1]
                 stats["cholqr"] = stats.get("cholqr", 0) + 1
         torch.cuda.current_stream().synchronize()  # this solve's stream only
         stats["t_filter"] += time.perf_counter() - tf
@@ -323,7 +401,7 @@ def chfsi(A: BSR, k, precision="fp64", tol=1e-9, max_iter=300, guard=None, seed=
     scale_g = A.gershgorin_bound() * 1.0001 + 1e-12
     ub = min(scale_g, A.spectral_upper_bound() * 1.0001 + 1e-12)
     stats_ub = ub
-    stats = EigStats(spmm=0, rr=0, iters=0, t_filter=0.0, t_rr=0.0)
+    stats = EigStats(spmm=0, spmm_cols=0, rr=0, iters=0, t_filter=0.0, t_rr=0.0)
     gen = torch.Generator(device=dev)
     gen.manual_seed(int(seed))
 

@@ -290,3 +290,31 @@ def test_spectral_smoothing_vs_oracle():
     out = smoothing.vector_diffusion(c, 3.0, (w[:k], U), (wl[:kl], o.evecs_L[:, :kl]))
     ref2 = cpu.vector_diffusion(c, 3.0, (w[:k], U), (wl[:kl], o.evecs_L[:, :kl]), method="spectral")
     np.testing.assert_allclose(out, ref2, rtol=1e-10, atol=1e-12)
+
+
+@pytest.mark.parametrize("dtype", ["float64", "float32"])
+def test_per_column_degree_filter_equals_uniform_filters_per_group(dtype):
+    """_cheb_filter_cols with degrees (4, 4, ..., 9, ..., 13) per group of 8
+    columns equals, group by group, the one-degree filter _cheb_filter of
+    that group's degree on that group alone (the recurrence is column-wise;
+    the parity bookkeeping of the two iterate buffers is what is checked),
+    on the b = 2 ring connection Laplacian."""
+    import torch
+    from rvgp_amd.eigsolve import _cheb_filter, _cheb_filter_cols
+    n, theta = 400, 0.7
+    rp, c, blocks = ring_connection_laplacian(n, theta)
+    A = _bsr_from_blocks(rp, c, blocks, 2)
+    dt = getattr(torch, dtype)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(5)
+    X0 = torch.randn((2 * n, 48), generator=g, device="cuda", dtype=torch.float64).to(dt)
+    ms = np.repeat([4, 4, 9, 10, 10, 13], 8)
+    a, b, a0 = 0.05, 4.0, 0.0
+    Y = _cheb_filter_cols(A, X0.clone(), ms, a, b, a0)
+    for grp in range(6):
+        sl = slice(8 * grp, 8 * grp + 8)
+        ref = _cheb_filter(A, X0[:, sl].contiguous(), int(ms[8 * grp]), a, b, a0)
+        tol = 1e-12 if dtype == "float64" else 1e-5
+        err = float(((Y[:, sl] - ref).abs().max() / ref.abs().max()).item())
+        assert err < tol, (grp, err)
+

@@ -58,3 +58,19 @@ def test_dense_operand_stride_rules():
         _ld(torch.zeros(1, 4).expand(6, 4))
     with pytest.raises(ValueError):
         _ld(torch.zeros(1, 1).expand(6, 1))
+
+
+def test_column_degrees_are_monotone_grouped_and_bounded():
+    """_column_degrees: non-decreasing along the block, constant over groups
+    of 8 columns, within [4, m], and m for the guard columns beyond kk; a
+    column far below the cut with a small residual needs fewer steps than one
+    near the cut."""
+    from rvgp_amd.eigsolve import _column_degrees
+    theta = np.linspace(0.0, 0.95, 40)
+    res = np.full(40, 1e-3)
+    res[:8] = 1e-6
+    ms = _column_degrees(theta, res, 1e-3, 56, 40, 1.0, 20.0, 30, 1e-7)
+    assert ms.shape == (56,)
+    assert np.all(np.diff(ms) >= 0) and ms.min() >= 4 and ms.max() <= 30
+    assert np.all(ms.reshape(7, 8) == ms.reshape(7, 8)[:, :1])
+    assert np.all(ms[40:] == 30) and ms[0] < ms[39]

@@ -257,21 +257,22 @@ def _run_phase(A, X, k, ub, wdt, ptol, max_iter, max_ratio, lock, fresh, stats, 
     """Filter + Rayleigh-Ritz iterations on block X (work dtype wdt) until the
     k wanted pairs have residual/ub < ptol.  With ``lock``, converged leading
     pairs are frozen and deflated out of the filtered block.  ``ritz`` =
-    (theta, worst residual / ub) of X when X already is a Ritz basis (the
-    previous phase's result): the phase then opens with a filter pass instead
-    of a Rayleigh-Ritz step, which would only re-derive those values.
+    (theta, worst residual, per-column residuals or None) of X when X
+    already is a Ritz basis (the previous phase's result; residuals relative
+    to ``scale``): the phase then opens with a filter pass instead of a
+    Rayleigh-Ritz step, which would only re-derive those values.
     ``ub`` bounds the spectrum for the filters; residuals are measured
     relative to ``scale`` (default ub)."""
     scale = ub if scale is None else scale
     p = X.shape[1]
     history = []
-    locked_res = 0.0  # largest residual / scale among the locked pairs
+    locked_res = np.zeros(0)  # residual / scale of each locked pair when it was locked
     Xl = X[:, :0]
     thl = torch.zeros(0, dtype=torch.float64, device=X.device)
     Y = X
     it = 0
     if ritz is not None:
-        th0, worst0 = ritz
+        th0, worst0, res0 = ritz
         tf = time.perf_counter()
         cut, low = float(th0.max().item()), float(th0.min().item())
         m, m_need = _degree(torch.sort(th0).values, min(k, th0.numel()), cut, ub, max_ratio,
@@ -281,7 +282,7 @@ def _run_phase(A, X, k, ub, wdt, ptol, max_iter, max_ratio, lock, fresh, stats, 
                   file=sys.stderr, flush=True)
         if ADAPTIVE_DEGREES:
             th_h = th0.cpu().numpy()
-            ms = _column_degrees(th_h, None, worst0, X.shape[1], min(k, th_h.size), cut, ub, m, ptol)
+            ms = _column_degrees(th_h, res0, worst0, X.shape[1], min(k, th_h.size), cut, ub, m, ptol)
             Y = _cheb_filter_cols(A, X, ms, cut, ub, low)
             stats["spmm_cols"] += int(ms.sum())
         else:
@@ -306,7 +307,11 @@ def _run_phase(A, X, k, ub, wdt, ptol, max_iter, max_ratio, lock, fresh, stats, 
             print(f"[chfsi {tag}] it={it} locked={Xl.shape[1]} active={Xa.shape[1]} "
                   f"worst_res={worst:.3e} spmm={stats['spmm']}", file=sys.stderr, flush=True)
         stats["last_worst"] = worst
-        stats["phase_residual"] = max(worst, locked_res)
+        stats["phase_residual"] = max([worst, *locked_res])
+        res_h = res.cpu().numpy()
+        # per-column residuals of the block this phase returns (locked pairs
+        # as locked), for the next phase's per-column filter degrees
+        stats["last_res"] = np.concatenate([locked_res, res_h])
         if kk >= need and worst < ptol:
             X = torch.cat([Xl, Xa], 1)
             return X, torch.cat([thl, tha]), it, True
@@ -320,7 +325,6 @@ def _run_phase(A, X, k, ub, wdt, ptol, max_iter, max_ratio, lock, fresh, stats, 
             X = torch.cat([Xl, Xa], 1)
             stats["unconverged"] = worst
             return X, torch.cat([thl, tha]), it, False
-        res_h = res.cpu().numpy()
         c = 0
         if lock:
             while c < kk and res_h[c] < ptol:
@@ -328,7 +332,7 @@ def _run_phase(A, X, k, ub, wdt, ptol, max_iter, max_ratio, lock, fresh, stats, 
             c = min(c, max(0, Xa.shape[1] - 8))  # keep a working block
             c -= c % 8  # block widths stay multiples of 8: 16-byte SpMM rows
             if c > 0:
-                locked_res = max(locked_res, float(res_h[:c].max()))
+                locked_res = np.concatenate([locked_res, res_h[:c]])
                 Xl = torch.cat([Xl, Xa[:, :c]], 1).contiguous()
                 thl = torch.cat([thl, tha[:c]])
                 Xa, tha = Xa[:, c:].contiguous(), tha[c:]
@@ -412,7 +416,7 @@ def chfsi(A: BSR, k, precision="fp64", tol=1e-9, max_iter=300, guard=None, seed=
     X = X0.contiguous() if X0 is not None else fresh(p, torch.float64)
     # a random block opens with a filter pass bounded by where its Ritz
     # values would lie, instead of a Rayleigh-Ritz step on the unfiltered block
-    opening = None if X0 is not None else (_random_block_bracket(A, p), 1.0)
+    opening = None if X0 is not None else (_random_block_bracket(A, p), 1.0, None)
     if precision == "mixed":
         # FP32 filter body with locking; hands over at 1e-5, a few times above
         # the FP32 residual floor (a phase that reaches the floor first
@@ -427,7 +431,8 @@ def chfsi(A: BSR, k, precision="fp64", tol=1e-9, max_iter=300, guard=None, seed=
         # ~1e-5, so a few filter passes finish the pairs
         X, theta, it, done = _run_phase(A, X.double().contiguous(), k, ub, torch.float64, tol, 50,
                                         1e4, True, fresh, stats, verbose, "fp64-polish",
-                                        ritz=(theta, stats["last_worst"]), scale=scale_g)
+                                        ritz=(theta, stats["last_worst"], stats["last_res"]),
+                                        scale=scale_g)
         if not done:
             # an unlocked Rayleigh-Ritz over the whole block orders and refines
             X, theta, _, _ = _run_phase(A, X, k, ub, torch.float64, tol, 3, FP64_MAX_RATIO, False, fresh,

@@ -144,18 +144,32 @@ __global__ void colsumsq_part_kernel(int M, int k, int parts, const T* __restric
         ((s[0] + s[1]) + (s[2] + s[3])) + ((s[4] + s[5]) + (s[6] + s[7]));
 }
 
-__global__ void colsumsq_reduce_kernel(int k, int parts, const double* __restrict__ part,
-                                       double* __restrict__ out) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= k) return;
-    double s[4] = {0.0, 0.0, 0.0, 0.0};
-    int p = 0;
-    for (; p + 3 < parts; p += 4) {
+// out[j] = sum of the slice partials: 16 wavefronts per 64 columns,
+// wavefront w summing the slices p = w (mod 16), the 16 partial sums then
+// added in wavefront order (a fixed order).
+__global__ __launch_bounds__(1024) void colsumsq_reduce_kernel(int k, int parts,
+                                                               const double* __restrict__ part,
+                                                               double* __restrict__ out) {
+    __shared__ double sp[16][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int j = blockIdx.x * 64 + lane;
+    // four slices in flight per step: the loads do not wait on each other
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
+    if (j < k) {
+        int q = w;
+        for (; q + 48 < parts; q += 64) {
 #pragma unroll
-        for (int t = 0; t < 4; ++t) s[t] += part[(size_t)(p + t) * k + j];
+            for (int t = 0; t < 4; ++t) a[t] += part[(size_t)(q + 16 * t) * k + j];
+        }
+        for (int t = 0; q < parts; q += 16, ++t) a[t] += part[(size_t)q * k + j];
     }
-    for (int t = 0; p < parts; ++p, ++t) s[t] += part[(size_t)p * k + j];
-    out[j] = (s[0] + s[1]) + (s[2] + s[3]);
+    sp[w][lane] = (a[0] + a[1]) + (a[2] + a[3]);
+    __syncthreads();
+    if (w == 0 && j < k) {
+        double t = 0.0;
+        for (int q = 0; q < 16; ++q) t += sp[q][lane];
+        out[j] = t;
+    }
 }
 
 // Dot product out = sum_i x_i y_i in f64, fixed order: P slices of 2048
@@ -375,7 +389,7 @@ static int colsumsq_launch(int f64, int M, int k, const void* Z, int ldz, double
         hipLaunchKernelGGL(colsumsq_part_kernel<float>, g, dim3(256), 0, as_stream(stream), M, k,
                            parts, static_cast<const float*>(Z), ldz, part);
     RVGP_LAUNCH_CHECK();
-    hipLaunchKernelGGL(colsumsq_reduce_kernel, dim3(cdiv(k, 256)), dim3(256), 0, as_stream(stream), k,
+    hipLaunchKernelGGL(colsumsq_reduce_kernel, dim3(cdiv(k, 64)), dim3(1024), 0, as_stream(stream), k,
                        parts, part, out);
     RVGP_LAUNCH_CHECK();
     return RVGP_OK;

@@ -55,17 +55,42 @@ __global__ __launch_bounds__(256) void ritz_res_part_kernel(int M, int k, int pa
     o[1] = (sx[0] + sx[1]) + (sx[2] + sx[3]);
 }
 
-__global__ __launch_bounds__(256) void ritz_res_reduce_kernel(int k, int parts,
-                                                              const double* __restrict__ part,
-                                                              double* __restrict__ res) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= k) return;
-    double sr = 0.0, sx = 0.0;
-    for (int p = 0; p < parts; ++p) {
-        sr += part[((size_t)p * k + j) * 2];
-        sx += part[((size_t)p * k + j) * 2 + 1];
+// res[j] = sqrt(sum_p r / sum_p x) over the slices: 16 wavefronts per 64
+// columns, wavefront w summing the slices p = w (mod 16), the 16 partial
+// sums then added in wavefront order (a fixed order).
+__global__ __launch_bounds__(1024) void ritz_res_reduce_kernel(int k, int parts,
+                                                               const double* __restrict__ part,
+                                                               double* __restrict__ res) {
+    __shared__ double sr[16][64], sx[16][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int j = blockIdx.x * 64 + lane;
+    // four slices in flight per step: the loads do not wait on each other
+    double ar[4] = {0.0, 0.0, 0.0, 0.0}, ax[4] = {0.0, 0.0, 0.0, 0.0};
+    if (j < k) {
+        int q = w;
+        for (; q + 48 < parts; q += 64) {
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                ar[t] += part[((size_t)(q + 16 * t) * k + j) * 2];
+                ax[t] += part[((size_t)(q + 16 * t) * k + j) * 2 + 1];
+            }
+        }
+        for (int t = 0; q < parts; q += 16, ++t) {
+            ar[t] += part[((size_t)q * k + j) * 2];
+            ax[t] += part[((size_t)q * k + j) * 2 + 1];
+        }
     }
-    res[j] = sx > 0.0 ? sqrt(sr / sx) : 0.0;
+    sr[w][lane] = (ar[0] + ar[1]) + (ar[2] + ar[3]);
+    sx[w][lane] = (ax[0] + ax[1]) + (ax[2] + ax[3]);
+    __syncthreads();
+    if (w == 0 && j < k) {
+        double tr = 0.0, tx = 0.0;
+        for (int q = 0; q < 16; ++q) {
+            tr += sr[q][lane];
+            tx += sx[q][lane];
+        }
+        res[j] = tx > 0.0 ? sqrt(tr / tx) : 0.0;
+    }
 }
 
 // out[perm[i] * b + a][j] = scale * X[i * b + a][j] / sqrt(nrm2[j]): one
@@ -116,7 +141,7 @@ extern "C" int rvgp_ritz_residuals(int f64, int M, int k, const void* X, int ldx
                            parts, static_cast<const float*>(X), ldx,
                            static_cast<const float*>(AX), ldax, theta, part);
     RVGP_LAUNCH_CHECK();
-    hipLaunchKernelGGL(ritz_res_reduce_kernel, dim3(cdiv(k, 256)), dim3(256), 0, as_stream(stream),
+    hipLaunchKernelGGL(ritz_res_reduce_kernel, dim3(cdiv(k, 64)), dim3(1024), 0, as_stream(stream),
                        k, parts, part, res);
     RVGP_LAUNCH_CHECK();
     return RVGP_OK;

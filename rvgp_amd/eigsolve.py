@@ -6,15 +6,19 @@ RVGP/geometry.py:66-80 (ARPACK Lanczos without shift-invert).
 Method: a block Chebyshev-filtered subspace iteration (Zhou & Saad 2007,
 "scaled" filter) with a Rayleigh-Ritz step.  Each iteration
 
-  1. filters the current block X through a degree-m Chebyshev polynomial of
-     A that is ~1 on [0, cut] and damped on [cut, ub] -- m fused BSR SpMMs
+  1. filters the current block X through a Chebyshev polynomial of A that
+     is ~1 on [0, cut] and damped on [cut, ub] -- fused BSR SpMMs
      Y <- (2s/e)(A - c) Y - s s' Y_prev (rvgp_bsr_spmm), the only work that
-     touches the n*d-sized operator;
+     touches the n*d-sized operator.  Each column gets the degree its own
+     Ritz value and residual need, so the later steps run on a shrinking
+     trailing slice of the block;
   2. Rayleigh-Ritz in FP64: G = Y^T Y and H = Y^T (A Y) on the matrix cores
-     (rvgp_gemm, split-K), G = V diag(g) V^T, B = V g^{-1/2} (drops
-     numerically dependent directions instead of failing like CholQR would),
-     eig(B^T H B) = Q diag(theta) Q^T, X = Y B Q and AX = (A Y) B Q;
-  3. residuals |AX - X theta| per column decide convergence.
+     (rvgp_gemm_sym, split-K), B = L^-T from the Cholesky factor of G (an
+     eigendecomposition of G instead when G is numerically singular, which
+     drops the dependent directions), eig(B^T H B) = Q diag(theta) Q^T,
+     X = Y B Q and AX = A X (one more SpMM);
+  3. residuals |AX - X theta| per column (rvgp_ritz_residuals) decide
+     convergence; converged leading pairs are locked.
 
 precision="fp64": everything in FP64 (the plain FP64 GPU form).
 precision="mixed": the filter and the tall products run on FP32 blocks with
@@ -25,8 +29,9 @@ p = k + guard) are solved on the GPU in FP64 by the library's own symmetric
 eigensolver (dense.syev: tridiagonalisation, bisection, inverse iteration),
 and the p x p products run on its matrix-core GEMM.
 
-Returns eigenvalues ascending and eigenvectors with unit 2-norm; the callers
-scale by sqrt(rows) as geometry.py:75 does.
+Returns eigenvalues ascending and eigenvectors with unit 2-norm times the
+caller's scale (sqrt(rows) as geometry.py:75), in the operator's original
+block-row numbering (rvgp_evecs_finish).
 """
 import math
 import sys
@@ -42,13 +47,13 @@ from .sparse import BSR, spmm
 # block sizes whose FP32 phase filters twice (Cholesky QR in between) while the
 # degree is capped by conditioning
 DOUBLE_FILTER_BLOCKS = (1,)
+# filter degrees chosen per column (shrinking trailing slices of the block)
+# rather than one degree for the whole block
+ADAPTIVE_DEGREES = True
 # largest amplification spread a filter may put across the block: FP32
 # blocks keep ~300 (the weakest directions then carry ~2e-5 relative
 # precision); FP64 blocks FP64_MAX_RATIO (the Rayleigh-Ritz Gram's condition
 # is its square, which the Cholesky basis must still resolve)
-# filter degrees chosen per column (shrinking trailing slices of the block)
-# rather than one degree for the whole block
-ADAPTIVE_DEGREES = True
 FP32_MAX_RATIO = 300.0
 FP64_MAX_RATIO = 1e4
 # the mixed form's FP32 phase hands over to FP64 at this residual (relative to
@@ -454,7 +459,8 @@ def chfsi(A: BSR, k, precision="fp64", tol=1e-9, max_iter=300, guard=None, seed=
     # the last phase's residuals (relative to the Gershgorin bound): the
     # active pairs' at its final Rayleigh-Ritz step, the locked pairs' when
     # they were locked
-    stats["residual"] = stats.pop("phase_residual")
+    stats["residual"] = float(stats.pop("phase_residual"))
+    stats.pop("last_res", None)
     stats["ub_filter"] = stats_ub
     stats["ub_gershgorin"] = scale_g
     evals = theta[:k].clone()

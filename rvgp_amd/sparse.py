@@ -26,13 +26,11 @@ class BSR:
         return self.vals32
 
     def gershgorin_bound(self):
-        """max_i sum_j |A_ij| >= lambda_max (host scalar)."""
-        v = self.vals64.abs().reshape(self.nnzb, self.b, self.b).sum(2)  # (nnzb, b) row sums
-        rows = torch.repeat_interleave(torch.arange(self.nb, device=v.device),
-                                       torch.diff(self.rowptr.long()))
-        acc = torch.zeros(self.nb, self.b, dtype=torch.float64, device=v.device)
-        acc.index_add_(0, rows, v)
-        return float(acc.max().item())
+        """max_i sum_j |A_ij| >= lambda_max (host scalar): the row sums are
+        |A| 1, one narrow block SpMM with the absolute values."""
+        absA = BSR(self.rowptr, self.cols, self.vals64.abs(), None, self.b)
+        ones = torch.ones(self.shape[0], dtype=torch.float64, device=self.vals64.device)
+        return float(spmm(absA, ones).max().item())
 
     def spectral_upper_bound(self, steps=40, seed=20111):
         """Upper bound of lambda_max for the Chebyshev filters and the heat

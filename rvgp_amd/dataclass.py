@@ -18,7 +18,7 @@ import torch
 from . import _lib, geometry
 from .eigsolve import chfsi
 from .ptu_dijkstra import connections_device, tangent_frames_device
-from .reorder import morton_perm, permute_bsr
+from .reorder import bsr_permutation, morton_perm, permute_bsr
 from .smoothing import lambda_min_lower, vector_diffusion_device, vector_diffusion_spectral_device
 from .sparse import BSR, graph_laplacian
 
@@ -66,7 +66,9 @@ class data:
         perm = perm if reorder else None
         t0 = time.perf_counter()
         Lb = graph_laplacian(G)
-        Lb_e = permute_bsr(Lb, perm) if reorder else Lb
+        # the renumbered pattern, shared by both Laplacians of the graph
+        plan = bsr_permutation(G.rowptr, G.cols, perm) if reorder else None
+        Lb_e = permute_bsr(Lb, perm, plan) if reorder else Lb
         sync()
         t["laplacian_L"] = time.perf_counter() - t0
         resL = {}
@@ -124,7 +126,7 @@ class data:
                 print('Compute eigendecompositions')
             t0 = time.perf_counter()
             kLc = N * dm if n_eigenpairs is None else min(int(n_eigenpairs), N * dm)
-            Lcb_e = permute_bsr(Lcb, perm) if reorder else Lcb
+            Lcb_e = permute_bsr(Lcb, perm, plan) if reorder else Lcb
             sync()
             t["permute_Lc"] = time.perf_counter() - t0
             t0 = time.perf_counter()

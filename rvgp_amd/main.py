@@ -217,7 +217,9 @@ def train_gp(data, train_ind=None, n_inducing_points=None, test_size=0.2, kernel
         eng.fit_lbfgs(maxiter=epochs, disp=verbose)
     elif optimizer == "adam":
         eng.adam_setup()
-        if use_graph and epochs > 1:
+        # a step that is already one launch gains nothing from a graph (the
+        # capture's warm-up step and instantiation cost more than they save)
+        if use_graph and epochs > 1 and not eng.fused():
             g = eng.capture_adam_step(lr=lr)
             for _ in range(int(epochs)):
                 g.replay()

@@ -37,23 +37,32 @@ def morton_perm(X, max_dims=6):
     return torch.sort(key, stable=True).indices
 
 
-def permute_bsr(A, perm):
-    """B = P A P^T for a BSR operator; perm[new] = old block row."""
-    from .sparse import BSR
-    dev = A.rowptr.device
-    nb, b = A.nb, A.b
+def bsr_permutation(rowptr, cols, perm):
+    """The structure of P A P^T for a block-CSR pattern (rowptr, cols) and
+    perm[new] = old block row: (new rowptr int32, new cols int32, order) with
+    order[e] = the stored block that lands at position e.  Operators on one
+    graph (the scalar and the connection Laplacian) share it."""
+    dev = rowptr.device
+    nb = rowptr.numel() - 1
     inv = torch.empty_like(perm)
     inv[perm] = torch.arange(nb, device=dev, dtype=perm.dtype)
-    rp = A.rowptr.long()
+    rp = rowptr.long()
     lens = rp[1:] - rp[:-1]
     rows = torch.repeat_interleave(torch.arange(nb, device=dev), lens)
     new_r = inv[rows]
-    new_c = inv[A.cols.long()]
+    new_c = inv[cols.long()]
     order = torch.sort(new_r * nb + new_c, stable=True).indices
-    new_lens = lens[perm]
     new_rp = torch.zeros(nb + 1, dtype=torch.int64, device=dev)
-    new_rp[1:] = torch.cumsum(new_lens, 0)
-    cols = new_c[order].to(torch.int32)
-    v64 = A.vals64.reshape(-1, b * b)[order].reshape(-1).contiguous()
-    v32 = None if A.vals32 is None else A.vals32.reshape(-1, b * b)[order].reshape(-1).contiguous()
-    return BSR(new_rp.to(torch.int32), cols, v64, v32, b)
+    new_rp[1:] = torch.cumsum(lens[perm], 0)
+    return new_rp.to(torch.int32), new_c[order].to(torch.int32), order
+
+
+def permute_bsr(A, perm, plan=None):
+    """B = P A P^T for a BSR operator; perm[new] = old block row.  plan: the
+    result of bsr_permutation for A's pattern, when already computed."""
+    from .sparse import BSR
+    rp, cols, order = plan if plan is not None else bsr_permutation(A.rowptr, A.cols, perm)
+    bb = A.b * A.b
+    v64 = A.vals64.reshape(-1, bb)[order].reshape(-1).contiguous()
+    v32 = None if A.vals32 is None else A.vals32.reshape(-1, bb)[order].reshape(-1).contiguous()
+    return BSR(rp, cols, v64, v32, A.b)

@@ -13,8 +13,10 @@ Prints ONE JSON line:
   value          wall time of the mixed-precision fit (s, lower is better)
   fp64_s         wall time of the FP64 GPU fit on the same input
   rel_err_pred   |f_mixed - f_fp64| / |f_fp64| over test-node predictions
-  roofline       dominant kernel (the FP32 block-SpMM of the Lc eigensolve):
-                 algorithmic bytes / HIP-event-timed duration vs 8 TB/s
+  roofline       dominant kernel (the block SpMM of the Lc eigensolve -- the
+                 FP64, FP32 or binary16 form with the largest summed time
+                 in the timed fit): algorithmic bytes / HIP-event-timed
+                 duration vs 8 TB/s
   cpu_baseline   the FP64 CPU oracle (oracle/) on a bounded sample of the
                  same workload, with the GPU time on that sample beside it
 """
@@ -92,11 +94,14 @@ def kernel_stats(rec):
     return out
 
 
-def traffic_from_profile():
-    """HBM bytes per launch of the dominant kernel from the committed
-    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, scaled by a plain copy
-    kernel), or None when no such summary is present."""
-    path = os.path.join(ROOT, "profiles", "spmm_f32_b3_pmc.txt")
+SPMM_SHORT = {"float64": "f64", "float32": "f32", "float16": "f16"}
+
+
+def traffic_from_profile(dtype="float32", b=3):
+    """HBM bytes per launch of the block SpMM of this dtype from the
+    committed rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, scaled by a
+    plain copy kernel), or None when no such summary is present."""
+    path = os.path.join(ROOT, "profiles", f"spmm_{SPMM_SHORT[dtype]}_b{b}_pmc.txt")
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -246,12 +251,13 @@ def gpu_same_sample(args):
 
 
 def roofline_only(args):
-    """Launch the dominant kernel alone at the headline shape: the FP32 block
-    SpMM (b = 3, buffer form) of the Lc eigensolve on the Morton-renumbered
-    operator, with the Chebyshev-step epilogue (Z read) at the starting block width
-    r = k + guard rounded up to a multiple of 8,
-    then a plain 1 GiB device copy (the reference for scaling PMC byte
-    counters).  Used under rocprofv3 --pmc; prints one JSON line."""
+    """Launch the dominant kernel alone at the headline shape: the block SpMM
+    (b = 3, buffer form; FP64, FP32 or binary16 storage by
+    --roofline-dtype) of the Lc eigensolve on the Morton-renumbered operator,
+    with the Chebyshev-step epilogue (Z read) at the starting block width
+    r = k + guard rounded up to a multiple of 8, then a plain 1 GiB device
+    copy (the reference for scaling PMC byte counters).  Used under
+    rocprofv3 --pmc; prints one JSON line."""
     import torch
     from rvgp_amd import _lib, geometry
     from rvgp_amd.ptu_dijkstra import connections_device, tangent_frames_device
@@ -270,7 +276,9 @@ def roofline_only(args):
     r = -(-(args.k_eig + max(8, int(math.ceil(0.15 * args.k_eig)))) // 8) * 8
     gen = torch.Generator(device="cuda")
     gen.manual_seed(0)
-    Xm = torch.randn((A.shape[0], r), generator=gen, device="cuda", dtype=torch.float32)
+    dt = getattr(torch, args.roofline_dtype)
+    Xm = torch.randn((A.shape[0], r), generator=gen, device="cuda",
+                     dtype=torch.float32).to(dt)
     Z = torch.randn_like(Xm)
     for _ in range(2):
         spmm(A, Xm, Y=Z, alpha=0.5, beta=0.1, Z=Z, gamma=-0.3)
@@ -283,13 +291,16 @@ def roofline_only(args):
     e1.record()
     torch.cuda.synchronize()
     us = 1e3 * e0.elapsed_time(e1) / reps
-    nbytes = spmm_bytes(A, r, 4, with_z=True)
+    s = Xm.element_size()
+    # binary16 blocks with f32 block values: the values count 4 bytes
+    nbytes = spmm_bytes(A, r, s, with_z=True) + (A.nnzb * A.b * A.b * 2 if s == 2 else 0)
     src = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
     dst = torch.empty_like(src)
     for _ in range(3):
         torch.mul(src, 1.0, out=dst)  # an elementwise kernel: reads 1 GiB, writes 1 GiB
     torch.cuda.synchronize()
-    print(json.dumps({"kernel": "spmm_buf<float,3,4>", "r": r, "nb": A.nb, "nnzb": A.nnzb,
+    kname = {8: "spmm_buf<double,3,2>", 4: "spmm_buf<float,3,4>", 2: "spmm_buf_h<3>"}[s]
+    print(json.dumps({"kernel": kname, "r": r, "nb": A.nb, "nnzb": A.nnzb,
                       "launches": reps + 2, "avg_us": us, "alg_bytes": nbytes,
                       "GBps": nbytes / (us * 1e3), "copy_bytes_each_way": 1 << 30}))
 
@@ -607,6 +618,8 @@ def main():
     p.add_argument("--cpu-k-eig", dest="cpu_k_eig", type=int, default=50)
     p.add_argument("--verbose", action="store_true")
     p.add_argument("--roofline-only", dest="roofline_only", action="store_true")
+    p.add_argument("--roofline-dtype", dest="roofline_dtype", default="float64",
+                   choices=sorted(SPMM_SHORT))
     p.add_argument("--config", default="fit",
                    choices=["fit", "batched", "multirhs", "sphere", "torus"],
                    help="fit: the headline full fit (default); batched: 256 fits on one "
@@ -710,17 +723,21 @@ def main():
         }
         del d64, gp64
 
-    # roofline of the dominant kernel: FP32 block SpMM (b = dim_man) of the
-    # Lc eigensolve, measured over every launch of that kernel in the timed fit
+    # roofline of the dominant kernel: the block SpMM (b = dim_man) of the Lc
+    # eigensolve -- of its FP64 (polish), FP32 and binary16 (opening passes)
+    # forms the one with the largest summed time in the timed fit -- measured
+    # over every launch of that kernel
     dm = d_mx.dim_man
-    cands = [f"spmm_buf[float32,b{dm}]", f"spmm_vec[float32,b{dm}]",
-             f"spmm_wide[float32,b{dm}]"]
+    cands = [f"{kind}[{dt},b{dm}]" for dt in SPMM_SHORT
+             for kind in (("spmm_buf_h",) if dt == "float16" else
+                          ("spmm_buf", "spmm_vec", "spmm_wide"))]
     key = max(cands, key=lambda c: kstats.get(c, {}).get("total_ms", 0.0))
-    ks = kstats.get(key, {"count": 0, "avg_us": 0.0, "GBps": 0.0})
+    ks = kstats.get(key, {"count": 0, "avg_us": 0.0, "GBps": 0.0, "total_ms": 0.0})
+    kdt = key.split("[")[1].split(",")[0]
     roof = {"kernel": key, "bound": "hbm", "achieved": round(ks["GBps"], 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(ks["GBps"] / HBM_PEAK_GBS, 4),
-            "traffic": traffic_from_profile(), "launches": ks["count"],
-            "avg_us": round(ks["avg_us"], 2)}
+            "traffic": traffic_from_profile(kdt, dm), "launches": ks["count"],
+            "avg_us": round(ks["avg_us"], 2), "total_ms": round(ks["total_ms"], 2)}
     gram = kstats.get("gemm[f32acc64]", None)
     step_util = gp_step_utilisation(gp_mx.engine)
 

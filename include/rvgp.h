@@ -168,6 +168,14 @@ int rvgp_bsr_spmm_f32(int nb, int b, const int* rowptr, const int* cols, const f
                       const float* X, int ldx, int r, float alpha, float beta, const float* Z,
                       int ldz, float gamma, float* Y, int ldy, float* acc, int ldacc,
                       float delta, void* stream);
+/* Half-storage form for the eigensolver's opening Chebyshev passes (the
+ * sparse products inside eigsh, geometry.py:73): X, Z, Y are IEEE binary16
+ * (uint16 storage), vals f32, all arithmetic f32, Y rounded to nearest even.
+ * r and ldx/ldy/ldz multiples of 8, X/Y/Z 16-byte aligned, X spanning
+ * < 2 GiB; no acc term. */
+int rvgp_bsr_spmm_f16(int nb, int b, const int* rowptr, const int* cols, const float* vals,
+                      const void* X, int ldx, int r, float alpha, float beta, const void* Z,
+                      int ldz, float gamma, void* Y, int ldy, void* stream);
 
 /* ---------------------------------------------------------------------
  * a10/a11  Vector-field frame algebra -- replaces geometry.project_to_manifold

@@ -38,6 +38,7 @@ SIGNATURES = {
     "rvgp_connections": (I, [P, I, I, I, I, P, P, I, P, P, P, P, P, P]),
     "rvgp_bsr_spmm_f64": (I, [I, I, P, P, P, P, I, I, D, D, P, I, D, P, I, P, I, D, P]),
     "rvgp_bsr_spmm_f32": (I, [I, I, P, P, P, P, I, I, F, F, P, I, F, P, I, P, I, F, P]),
+    "rvgp_bsr_spmm_f16": (I, [I, I, P, P, P, P, I, I, F, F, P, I, F, P, I, P]),
     "rvgp_gemm": (I, [I, I, I, I, I, I, D, P, I, P, I, P, D, P, I, I, P, S, P]),
     "rvgp_gemm_ws_bytes": (S, [I, I, I, I, I]),
     "rvgp_potrf": (I, [I, I, P, I, P, P]),

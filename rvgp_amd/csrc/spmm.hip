@@ -254,6 +254,69 @@ __global__ __launch_bounds__(256) void spmm_buf(
     }
 }
 
+// Half-storage form of spmm_buf: X, Z and Y are stored as IEEE binary16,
+// block values stay f32 and every product and sum is f32; only the output is
+// rounded to binary16 (round to nearest even).  A 16-byte gather moves 8
+// columns instead of 4, so a sweep issues half the gathers and half the
+// L1->L2 line requests of the f32 form for the same columns.  The
+// eigensolver runs its opening Chebyshev passes on this form, where the
+// residual targets (>= 1e-3) sit above binary16's 2^-11 rounding.
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+typedef float float8 __attribute__((ext_vector_type(8)));
+
+template <int B>
+__global__ __launch_bounds__(256) void spmm_buf_h(
+    int nb, const int* __restrict__ rowptr, const int* __restrict__ cols,
+    const float* __restrict__ vals, const _Float16* __restrict__ X, int ldx, int r, int gv,
+    int xbytes, float alpha, float beta, const _Float16* __restrict__ Z, int ldz, float gamma,
+    _Float16* __restrict__ Y, int ldy) {
+    constexpr int WPB = 4;
+    constexpr int BB = B * B;
+    const int lane = threadIdx.x & 63;
+    const int per = gridDim.x >> 3;  // XCD-aware row ranges, as in spmm_vec
+    const int lw = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+    const int row = __builtin_amdgcn_readfirstlane(lw * WPB + (threadIdx.x >> 6));
+    if (row >= nb) return;
+    const int c = (blockIdx.y * gv + lane) * 8;
+    if (!(lane < gv && c < r)) return;
+    const __amdgpu_buffer_rsrc_t xr =
+        __builtin_amdgcn_make_buffer_rsrc((void*)X, (short)0, xbytes, kBufRsrcWord3);
+    const int voff = c * 2;
+    const int rowbytes = ldx * 2;
+    float8 y[B];
+#pragma unroll
+    for (int a = 0; a < B; ++a) y[a] = float8(0.f);
+    const int e0 = rowptr[row], e1 = rowptr[row + 1];
+#pragma unroll 4
+    for (int e = e0; e < e1; ++e) {
+        const int j = cols[e];
+        const float* blk = vals + (size_t)e * BB;
+        float8 xb[B];
+#pragma unroll
+        for (int bb = 0; bb < B; ++bb)
+            xb[bb] = __builtin_convertvector(
+                __builtin_bit_cast(half8, __builtin_amdgcn_raw_buffer_load_b128(
+                                              xr, voff, (j * B + bb) * rowbytes, 0)),
+                float8);
+#pragma unroll
+        for (int a = 0; a < B; ++a)
+#pragma unroll
+            for (int bb = 0; bb < B; ++bb) y[a] += blk[a * B + bb] * xb[bb];
+    }
+#pragma unroll
+    for (int a = 0; a < B; ++a) {
+        const size_t rr = (size_t)(row * B + a);
+        float8 out = alpha * y[a];
+        if (beta != 0.f)
+            out += beta * __builtin_convertvector(
+                              *reinterpret_cast<const half8*>(X + rr * ldx + c), float8);
+        if (Z != nullptr && gamma != 0.f)
+            out += gamma * __builtin_convertvector(
+                               *reinterpret_cast<const half8*>(Z + rr * ldz + c), float8);
+        *reinterpret_cast<half8*>(Y + rr * ldy + c) = __builtin_convertvector(out, half8);
+    }
+}
+
 // Wide SpMM, latency-hiding form.  One wavefront per (block row, column
 // chunk).  The row's column indices and block values (up to 64 blocks at a
 // time) are fetched with ONE coalesced load per register (lane l holds entry
@@ -438,6 +501,47 @@ extern "C" int rvgp_bsr_spmm_f64(int nb, int b, const int* rowptr, const int* co
                                  double delta, void* stream) {
     return spmm_dispatch<double>(nb, b, rowptr, cols, vals, X, ldx, r, alpha, beta, Z, ldz, gamma,
                                  Y, ldy, acc, ldacc, delta, stream);
+}
+
+extern "C" int rvgp_bsr_spmm_f16(int nb, int b, const int* rowptr, const int* cols,
+                                 const float* vals, const void* X, int ldx, int r, float alpha,
+                                 float beta, const void* Z, int ldz, float gamma, void* Y, int ldy,
+                                 void* stream) {
+    RVGP_CHECK_ARG(nb >= 0 && r >= 8 && r % 8 == 0,
+                   "rvgp_bsr_spmm_f16: r must be a positive multiple of 8");
+    RVGP_CHECK_ARG(b >= 1 && b <= 3, "rvgp_bsr_spmm_f16: block size must be 1, 2 or 3");
+    RVGP_CHECK_ARG(ldx >= r && ldy >= r && ldx % 8 == 0 && ldy % 8 == 0 &&
+                       (Z == nullptr || (ldz >= r && ldz % 8 == 0)),
+                   "rvgp_bsr_spmm_f16: leading dimensions must be >= r and multiples of 8");
+    RVGP_CHECK_ARG(((uintptr_t)X | (uintptr_t)Y | (uintptr_t)Z) % 16 == 0,
+                   "rvgp_bsr_spmm_f16: X, Y and Z must be 16-byte aligned");
+    RVGP_CHECK_ARG(X != Y, "rvgp_bsr_spmm_f16: Y must not alias X");
+    const size_t xbytes = (size_t)nb * b * ldx * 2;
+    RVGP_CHECK_ARG(xbytes < ((size_t)1 << 31), "rvgp_bsr_spmm_f16: X must span < 2 GiB");
+    if (nb == 0) return RVGP_OK;
+    RVGP_CHECK_ARG(rowptr != nullptr && cols != nullptr && vals != nullptr && X != nullptr &&
+                       Y != nullptr,
+                   "rvgp_bsr_spmm_f16: null pointer");
+    // column groups of <= 64 8-column vectors, balanced, whole 128-byte lines
+    const int nv = r / 8;
+    int groups = (int)cdiv(nv, 64);
+    int gv = (nv + groups - 1) / groups;
+    gv = gv > 8 ? (gv + 7) / 8 * 8 : gv;
+    groups = (int)cdiv(nv, gv);
+    const dim3 g(8 * cdiv(cdiv(nb, 4), 8), groups);
+    hipStream_t s = as_stream(stream);
+    const _Float16* Xh = static_cast<const _Float16*>(X);
+    const _Float16* Zh = static_cast<const _Float16*>(Z);
+    _Float16* Yh = static_cast<_Float16*>(Y);
+#define RVGP_BUFH(BB)                                                                        \
+    hipLaunchKernelGGL((spmm_buf_h<BB>), g, dim3(256), 0, s, nb, rowptr, cols, vals, Xh, ldx, \
+                       r, gv, (int)xbytes, alpha, beta, Zh, ldz, gamma, Yh, ldy)
+    if (b == 1) RVGP_BUFH(1);
+    else if (b == 2) RVGP_BUFH(2);
+    else RVGP_BUFH(3);
+#undef RVGP_BUFH
+    RVGP_LAUNCH_CHECK();
+    return RVGP_OK;
 }
 
 extern "C" int rvgp_bsr_spmm_f32(int nb, int b, const int* rowptr, const int* cols,

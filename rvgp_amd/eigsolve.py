@@ -22,7 +22,9 @@ Method: a block Chebyshev-filtered subspace iteration (Zhou & Saad 2007,
 
 precision="fp64": everything in FP64 (the plain FP64 GPU form).
 precision="mixed": the filter and the tall products run on FP32 blocks with
-the Grams accumulated in FP64 (f32 -> f64 MFMA); once the FP32 residuals
+the Grams accumulated in FP64 (f32 -> f64 MFMA) -- the opening filter passes,
+whose residual targets lie above binary16 rounding, on binary16 blocks (f32
+arithmetic, half the gathered bytes); once the FP32 residuals
 stall at FP32 level, FP64 polish iterations (FP64 SpMM + FP64 RR) bring the
 wanted pairs to the FP64 tolerance.  The small projected problems (p x p,
 p = k + guard) are solved on the GPU in FP64 by the library's own symmetric
@@ -59,6 +61,14 @@ FP64_MAX_RATIO = 1e4
 # the mixed form's FP32 phase hands over to FP64 at this residual (relative to
 # the Gershgorin bound), a few times above the FP32 floor
 MIXED_HANDOVER = 1e-5
+# FP32-phase filter passes that start from a worst residual above this run
+# on binary16 blocks (rvgp_bsr_spmm_f16: f32 values and arithmetic, each
+# output rounded to 2^-11): on the 3-torus operators (n = 1e5, k = 500) the
+# last such pass still ends at the FP32 pass's residual (~5e-4); a pass
+# started below ~2e-3 on binary16 ends higher and costs later passes more
+# than it saved.  The FP32 passes and the FP64 polish after them set the
+# final accuracy, so this threshold moves only the time.
+HALF_ABOVE = 2e-3
 # a filter is given the degree that would cut the worst wanted residual by
 # NEED_SAFETY x the factor still needed
 NEED_SAFETY = 10.0
@@ -177,6 +187,34 @@ def _cheb_filter_cols(A, X, ms, a, b, a0):
     return out
 
 
+def _half_ok(X):
+    """Whether block X can be filtered on binary16 storage: whole 16-byte
+    rows of 8 columns, X spanning < 2 GiB at 2 bytes an entry."""
+    return X.shape[1] % 8 == 0 and X.shape[0] * X.shape[1] * 2 < 2 ** 31
+
+
+def _filter_pass(A, X, ms, m, a, b, a0, half=False):
+    """One Chebyshev filter pass over block X (degrees ms per column, or m
+    for every column when ms is None); returns the filtered block in X's
+    dtype.  half=True stores the block as binary16 for the pass, each column
+    first scaled to unit RMS: the filtered subspace is what the Rayleigh-Ritz
+    step uses, not the column scales.  The binary16 pass normalises the
+    filter at 0 instead of a0: the polynomial differs only by a constant
+    factor, and on a PSD operator no eigen-direction is then amplified at any
+    step (a random block's a0 is an estimate well inside the spectrum, below
+    which directions grow by up to ~1e4 in a pass -- past binary16's 65504)."""
+    if half:
+        a0 = min(a0, 0.0)
+        nrm = dense.colsumsq(X)
+        f = torch.rsqrt(torch.clamp(nrm / X.shape[0], min=1e-300)).to(X.dtype)
+        Xh = (X * f).to(torch.float16)
+        Yh = _filter_pass(A, Xh, ms, m, a, b, a0)
+        return Yh.to(X.dtype)
+    if ms is not None:
+        return _cheb_filter_cols(A, X, ms, a, b, a0)
+    return _cheb_filter(A, X, m, a, b, a0)
+
+
 def _column_degrees(theta, res, worst, ncols, kk, cut, ub, m, ptol, m_min=4):
     """Per-column filter degrees for a block whose first min(kk, len(theta))
     columns are wanted Ritz vectors (Ritz values theta, residuals res, or the
@@ -258,7 +296,7 @@ def _deflate(Y, Xl):
 
 
 def _run_phase(A, X, k, ub, wdt, ptol, max_iter, max_ratio, lock, fresh, stats, verbose, tag,
-               stall_exit=False, m_max=60, ritz=None, scale=None):
+               stall_exit=False, m_max=60, ritz=None, scale=None, half_above=None):
     """Filter + Rayleigh-Ritz iterations on block X (work dtype wdt) until the
     k wanted pairs have residual/ub < ptol.  With ``lock``, converged leading
     pairs are frozen and deflated out of the filtered block.  ``ritz`` =
@@ -267,7 +305,8 @@ def _run_phase(A, X, k, ub, wdt, ptol, max_iter, max_ratio, lock, fresh, stats, 
     to ``scale``): the phase then opens with a filter pass instead of a
     Rayleigh-Ritz step, which would only re-derive those values.
     ``ub`` bounds the spectrum for the filters; residuals are measured
-    relative to ``scale`` (default ub)."""
+    relative to ``scale`` (default ub).  Filter passes that start from a
+    worst residual above ``half_above`` run on binary16 blocks."""
     scale = ub if scale is None else scale
     p = X.shape[1]
     history = []
@@ -282,18 +321,18 @@ def _run_phase(A, X, k, ub, wdt, ptol, max_iter, max_ratio, lock, fresh, stats, 
         cut, low = float(th0.max().item()), float(th0.min().item())
         m, m_need = _degree(torch.sort(th0).values, min(k, th0.numel()), cut, ub, max_ratio,
                             m_max=m_max, need=NEED_SAFETY * worst0 / ptol)
+        half = half_above is not None and worst0 > half_above and _half_ok(X)
         if verbose:
-            print(f"[chfsi {tag}] opening filter m={m} m_need={m_need} worst={worst0:.3e}",
-                  file=sys.stderr, flush=True)
+            print(f"[chfsi {tag}] opening filter m={m} m_need={m_need} worst={worst0:.3e}"
+                  f"{' binary16' if half else ''}", file=sys.stderr, flush=True)
+        ms = None
         if ADAPTIVE_DEGREES:
             th_h = th0.cpu().numpy()
             ms = _column_degrees(th_h, res0, worst0, X.shape[1], min(k, th_h.size), cut, ub, m, ptol)
-            Y = _cheb_filter_cols(A, X, ms, cut, ub, low)
-            stats["spmm_cols"] += int(ms.sum())
-        else:
-            Y = _cheb_filter(A, X, m, cut, ub, low)
-            stats["spmm_cols"] += m * X.shape[1]
+        Y = _filter_pass(A, X, ms, m, cut, ub, low, half)
+        stats["spmm_cols"] += int(ms.sum()) if ms is not None else m * X.shape[1]
         stats["spmm"] += m
+        stats["half_passes"] += int(half)
         torch.cuda.current_stream().synchronize()
         stats["t_filter"] += time.perf_counter() - tf
         stats["iters"] += 1
@@ -349,19 +388,19 @@ def _run_phase(A, X, k, ub, wdt, ptol, max_iter, max_ratio, lock, fresh, stats, 
         kk = min(k - Xl.shape[1], tha.numel())
         m, m_need = _degree(tha, max(kk, 1), cut, ub, max_ratio, m_max=m_max,
                             need=NEED_SAFETY * worst / ptol if kk > 0 else None)
+        half = half_above is not None and worst > half_above and _half_ok(Xa)
         if verbose:
-            print(f"[chfsi {tag}] filter m={m} m_need={m_need} cut={cut:.4e} low={low:.4e}",
-                  file=sys.stderr, flush=True)
+            print(f"[chfsi {tag}] filter m={m} m_need={m_need} cut={cut:.4e} low={low:.4e}"
+                  f"{' binary16' if half else ''}", file=sys.stderr, flush=True)
         tf = time.perf_counter()
+        ms = None
         if ADAPTIVE_DEGREES:
             ms = _column_degrees(tha.cpu().numpy(), res_h[c:], worst, Xa.shape[1], max(kk, 0), cut,
                                  ub, m, ptol)
-            Y = _cheb_filter_cols(A, Xa, ms, cut, ub, low)
-            stats["spmm_cols"] += int(ms.sum())
-        else:
-            Y = _cheb_filter(A, Xa, m, cut, ub, low)
-            stats["spmm_cols"] += m * Xa.shape[1]
+        Y = _filter_pass(A, Xa, ms, m, cut, ub, low, half)
+        stats["spmm_cols"] += int(ms.sum()) if ms is not None else m * Xa.shape[1]
         stats["spmm"] += m
+        stats["half_passes"] += int(half)
         if (A.b in DOUBLE_FILTER_BLOCKS and wdt == torch.float32 and Xl.shape[1] == 0
                 and m_need is not None
                 and m_need > m):
@@ -377,7 +416,7 @@ def _run_phase(A, X, k, ub, wdt, ptol, max_iter, max_ratio, lock, fresh, stats, 
             Q = _cholqr(Y)
             if Q is not None:
                 m2 = max(4, min(m, m_need - m))
-                Y = _cheb_filter(A, Q, m2, cut, ub, low)
+                Y = _filter_pass(A, Q, None, m2, cut, ub, low, half)
                 stats["spmm"] += m2
                 stats["spmm_cols"] += m2 * Q.shape[1]
                 stats["cholqr"] = stats.get("cholqr", 0) + 1
@@ -410,7 +449,7 @@ def chfsi(A: BSR, k, precision="fp64", tol=1e-9, max_iter=300, guard=None, seed=
     scale_g = A.gershgorin_bound() * 1.0001 + 1e-12
     ub = min(scale_g, A.spectral_upper_bound() * 1.0001 + 1e-12)
     stats_ub = ub
-    stats = EigStats(spmm=0, spmm_cols=0, rr=0, iters=0, t_filter=0.0, t_rr=0.0)
+    stats = EigStats(spmm=0, spmm_cols=0, rr=0, iters=0, t_filter=0.0, t_rr=0.0, half_passes=0)
     gen = torch.Generator(device=dev)
     gen.manual_seed(int(seed))
 
@@ -428,8 +467,8 @@ def chfsi(A: BSR, k, precision="fp64", tol=1e-9, max_iter=300, guard=None, seed=
         # stalls there)
         X, theta, it, _ = _run_phase(A, X.to(torch.float32).contiguous(), k, ub, torch.float32,
                                      MIXED_HANDOVER, max_iter, FP32_MAX_RATIO, True, fresh, stats,
-                                     verbose, "fp32",
-                                     stall_exit=True, scale=scale_g, ritz=opening)
+                                     verbose, "fp32", stall_exit=True, scale=scale_g,
+                                     ritz=opening, half_above=HALF_ABOVE)
         stats["iters_fp32"] = it
         stats["t_fp32"] = time.perf_counter() - t0
         # FP64 polish with locking: the FP32 block is already accurate to

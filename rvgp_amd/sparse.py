@@ -96,7 +96,8 @@ class BSR:
 
 def spmm(A, X, Y=None, alpha=1.0, beta=0.0, Z=None, gamma=0.0, acc=None, delta=0.0):
     """Y = alpha A X + beta X + gamma Z ; acc += delta Y.  X is (nb*b, r) or
-    (nb*b,); dtype float64 or float32 selects the kernel and A's values."""
+    (nb*b,); dtype float64 or float32 selects the kernel and A's values;
+    float16 selects the half-storage form (f32 values and arithmetic)."""
     vec = X.dim() == 1
     X2 = X.reshape(-1, 1) if vec else X
     assert X2.shape[0] == A.shape[0], (X2.shape, A.shape)
@@ -108,6 +109,8 @@ def spmm(A, X, Y=None, alpha=1.0, beta=0.0, Z=None, gamma=0.0, acc=None, delta=0
     Y2 = Y.reshape(-1, 1) if Y.dim() == 1 else Y
     Z2 = None if Z is None else (Z.reshape(-1, 1) if Z.dim() == 1 else Z)
     A2 = None if acc is None else (acc.reshape(-1, 1) if acc.dim() == 1 else acc)
+    if X2.dtype == torch.float16:
+        return _spmm_f16(A, X2, Y, Y2, alpha, beta, Z2, gamma, acc)
     f64 = X2.dtype == torch.float64
     vals = A.vals(X2.dtype)
     ev = instrument.begin()
@@ -127,6 +130,24 @@ def spmm(A, X, Y=None, alpha=1.0, beta=0.0, Z=None, gamma=0.0, acc=None, delta=0
             kind = "spmm_buf" if A.nb * A.b * X2.stride(0) * s < 2 ** 31 else "spmm_vec"
         instrument.end(ev, kind, (str(X2.dtype).replace("torch.", ""), A.b), nbytes,
                        2.0 * A.nnzb * A.b * A.b * r)
+    return Y
+
+
+def _spmm_f16(A, X2, Y, Y2, alpha, beta, Z2, gamma, acc):
+    """Half-storage SpMM (binary16 X, Z, Y; f32 values and arithmetic):
+    r and the row strides multiples of 8, no acc term."""
+    if acc is not None:
+        raise ValueError("spmm: the binary16 form has no acc term")
+    ev = instrument.begin()
+    r = X2.shape[1]
+    _lib.call("rvgp_bsr_spmm_f16", A.nb, A.b, _lib.ptr(A.rowptr), _lib.ptr(A.cols),
+              _lib.ptr(A.vals(torch.float32)), _lib.ptr(X2), X2.stride(0), r, float(alpha),
+              float(beta), _lib.ptr(Z2), Z2.stride(0) if Z2 is not None else r, float(gamma),
+              _lib.ptr(Y2), Y2.stride(0), _lib.stream())
+    if ev is not None:
+        nbytes = spmm_bytes(A, r, 2, Z2 is not None and gamma != 0.0) \
+            + A.nnzb * A.b * A.b * 2  # values are f32: 4 bytes, not 2
+        instrument.end(ev, "spmm_buf_h", ("float16", A.b), nbytes, 2.0 * A.nnzb * A.b * A.b * r)
     return Y
 
 

@@ -40,6 +40,13 @@ def test_arg_validation_reports_value_errors_without_a_gpu():
     rc = L.rvgp_bsr_spmm_f64(10, 4, None, None, None, None, 1, 1, 1.0, 0.0, None, 1, 0.0, None, 1,
                              None, 1, 0.0, None)
     assert rc == _lib.RVGP_EINVAL
+    # binary16 SpMM: r a multiple of 8, 16-byte aligned operands
+    rc = L.rvgp_bsr_spmm_f16(10, 3, None, None, None, None, 12, 12, 1.0, 0.0, None, 12, 0.0,
+                             None, 12, None)
+    assert rc == _lib.RVGP_EINVAL and b"multiple of 8" in L.rvgp_last_error()
+    rc = L.rvgp_bsr_spmm_f16(10, 3, None, None, None, 8, 16, 16, 1.0, 0.0, None, 16, 0.0,
+                             64, 16, None)
+    assert rc == _lib.RVGP_EINVAL and b"aligned" in L.rvgp_last_error()
     rc = L.rvgp_gemm(7, 0, 0, 1, 1, 1, 1.0, None, 1, None, 1, None, 0.0, None, 1, 1, None, 0, None)
     assert rc == _lib.RVGP_EINVAL
     rc = L.rvgp_gauges(None, 10, 65, None, 10, 1, 1, 1, None, None, None, None)

@@ -342,6 +342,52 @@ def test_bsr_spmm(b, r):
         assert np.abs(Y2.cpu().double().numpy() - As @ X).max() < tol * np.abs(As @ X).max() * 10
 
 
+@pytest.mark.parametrize("b", [1, 2, 3])
+@pytest.mark.parametrize("r", [8, 136, 520])
+def test_bsr_spmm_binary16(b, r):
+    """rvgp_bsr_spmm_f16 (binary16 X, Z, Y; f32 block values and arithmetic)
+    vs the FP64 product of the same binary16 inputs and f32 values.
+    Tolerance: the output's one rounding to binary16 (2^-11 of |ref|) plus
+    the f32 accumulation (1e-6 of the sum of the absolute terms).  Covers
+    one and two column groups (r = 520: 65 8-column vectors), a column slice
+    with the parent's row stride (the per-column-degree filter's form) and
+    Y written over Z (the filter recurrence's form)."""
+    import torch
+    from rvgp_amd import _lib
+    from rvgp_amd.sparse import BSR, spmm
+    rng = np.random.default_rng(b * 1000 + r)
+    n = 700
+    x = sphere(n, b + r)
+    rp, c = cpu.sym_csr(cpu.knn(x, 8))
+    vals = rng.normal(size=(len(c), b, b))
+    A = BSR(torch.as_tensor(rp, device="cuda"), torch.as_tensor(c, device="cuda"),
+            torch.as_tensor(vals.reshape(-1), device="cuda"), None, b)
+    A32 = BSR(A.rowptr, A.cols, A.vals(torch.float32).double(), None, b).to_scipy()
+    Xh = torch.as_tensor(rng.normal(size=(n * b, r)), device="cuda").half()
+    Zh = torch.as_tensor(rng.normal(size=(n * b, r)), device="cuda").half()
+    X, Z = Xh.double().cpu().numpy(), Zh.double().cpu().numpy()
+
+    def check(Y, ref, absterms):
+        err = np.abs(Y.double().cpu().numpy() - ref)
+        assert (err <= 2.0 ** -11 * np.abs(ref) + 1e-6 * absterms).all(), err.max()
+
+    Y = spmm(A, Xh, alpha=0.7, beta=-0.3, Z=Zh, gamma=1.5)
+    assert Y.dtype == torch.float16
+    check(Y, 0.7 * (A32 @ X) - 0.3 * X + 1.5 * Z,
+          0.7 * (abs(A32) @ np.abs(X)) + 0.3 * np.abs(X) + 1.5 * np.abs(Z))
+    if r > 8:
+        # trailing column slice (row stride r), written over Z's slice
+        Zs = Zh.clone()
+        Ys = spmm(A, Xh[:, 8:], Y=Zs[:, 8:], alpha=-0.4, beta=0.25, Z=Zs[:, 8:], gamma=-1.0)
+        ref = -0.4 * (A32 @ X[:, 8:]) + 0.25 * X[:, 8:] - Z[:, 8:]
+        check(Ys, ref, 0.4 * (abs(A32) @ np.abs(X[:, 8:])) + 0.25 * np.abs(X[:, 8:])
+              + np.abs(Z[:, 8:]))
+        assert torch.equal(Zs[:, :8], Zh[:, :8])
+    # r not a multiple of 8 is refused (no silent widening)
+    with pytest.raises(ValueError, match="multiple of 8"):
+        spmm(A, Xh[:, :r - 4].contiguous())
+
+
 
 def test_rw_connection_laplacian_uses_networkx_degrees():
     """compute_connection_laplacian(G, R, 'rw') (geometry.py:45-50): D^-1 Lc

@@ -58,6 +58,9 @@ ADAPTIVE_DEGREES = True
 # is its square, which the Cholesky basis must still resolve)
 FP32_MAX_RATIO = 300.0
 FP64_MAX_RATIO = 1e4
+# the same cap for the mixed form's FP64 polish (its block is already
+# accurate to the FP32 floor)
+POLISH_MAX_RATIO = 1e4
 # the mixed form's FP32 phase hands over to FP64 at this residual (relative to
 # the Gershgorin bound), a few times above the FP32 floor
 MIXED_HANDOVER = 1e-5
@@ -474,7 +477,7 @@ def chfsi(A: BSR, k, precision="fp64", tol=1e-9, max_iter=300, guard=None, seed=
         # FP64 polish with locking: the FP32 block is already accurate to
         # ~1e-5, so a few filter passes finish the pairs
         X, theta, it, done = _run_phase(A, X.double().contiguous(), k, ub, torch.float64, tol, 50,
-                                        1e4, True, fresh, stats, verbose, "fp64-polish",
+                                        POLISH_MAX_RATIO, True, fresh, stats, verbose, "fp64-polish",
                                         ritz=(theta, stats["last_worst"], stats["last_res"]),
                                         scale=scale_g)
         if not done:

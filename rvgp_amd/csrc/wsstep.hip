@@ -11,7 +11,8 @@
 // dataflow and waits for the tiles it reads through per-task flags:
 //   L(j,j)  C = A(j,j) - sum_{p<j} L(j,p) L(j,p)^T; L = chol(C), D(j) =
 //           L(j,j)^-1, and the tile below, L(j+1,j) = C(j+1,j) D(j)^T, so
-//           that the next diagonal task is one hand-off away
+//           that the next diagonal task is one hand-off away (one wave
+//           factors; the other three apply C(j+1,j)'s last update meanwhile)
 //   L(i,j)  i > j+1: C = A(i,j) - sum_{p<j} L(i,p) L(j,p)^T; L = C D(j)^T
 //   X(i,j)  i > j: X = -D(i) sum_{p=j}^{i-1} L(i,p) X(p,j),  X(p,p) = D(p)
 //   filter  S, dS and sum log S (no dependencies)
@@ -147,6 +148,24 @@ __device__ void wait_for(const int* flag, int epoch, int* ctl) {
     __syncthreads();
 }
 
+// wait_for for one wave (the others of its workgroup keep working): its lane
+// 0 polls, then the whole wave acquires.
+__device__ void wait_for_wave(const int* flag, int epoch, int* ctl) {
+    if ((threadIdx.x & 63) == 0) {
+        int n = 0;
+        while ((int)(ld_flag(flag) - epoch) < 0) {
+            ++n;
+            if (n > kSpinLimit || ((n & 1023) == 0 && ld_flag(ctl + 1) != 0)) {
+                st_flag(ctl + 1, 1);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+
 // Every wave waits for its own stores to reach the L2 (vmcnt 0); after the
 // barrier one lane's agent-scope release fence writes the L2 back -- the
 // workgroup's waves share one CU and so one L2 -- and raises the flag.
@@ -181,6 +200,14 @@ __device__ __forceinline__ void quad_store(const dbl4& acc, double* C, int ldc, 
 #pragma unroll
     for (int r = 0; r < 4; ++r)
         C[(qi * 16 + (lane >> 4) + 4 * r) * ldc + qj * 16 + (lane & 15)] = acc[r];
+}
+
+__device__ __forceinline__ dbl4 quad_load(const double* C, int ldc, int qi, int qj, int lane) {
+    dbl4 v;
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+        v[r] = C[(qi * 16 + (lane >> 4) + 4 * r) * ldc + qj * 16 + (lane & 15)];
+    return v;
 }
 
 // 16 x 16 products on one wave: acc += sgn * P Q^T (TRANS) or sgn * P Q,
@@ -380,6 +407,8 @@ __device__ void task_diag(const StepArgs& a, const Layout& g, int j, int epoch, 
         }
     }
     quad_store(acc, Cs, LDT, qi, qj, lane);
+    double* C1 = Cs + 3 * TB * LDT + 16 * 17;  // 32 x 33: C(j+1,j), past wave 0's scratch
+    if (has1) quad_store(acc1, C1, LDT, qi, qj, lane);
     __syncthreads();
     // Blocked factor and inverse of the tile on wave 0 (2 x 2 blocks of 16):
     //   L11 = chol(C11), D11 = L11^-1;  L21 = C21 D11^T;
@@ -414,18 +443,24 @@ __device__ void task_diag(const StepArgs& a, const Layout& g, int j, int epoch, 
         wave_lds_sync();
         bad = (int)wave_max((double)bad);
         if (bad && lane == 0) atomicOr(ctl + 2, 1);
+    } else if (has1 && j > 0) {
+        // meanwhile waves 1-3 apply C(j+1,j)'s last update, from L(j+1,j-1),
+        // quadrant by quadrant in LDS (wave 1: (0,0) and (1,1); wave 2: (0,1);
+        // wave 3: (1,0)) -- the accumulation the owner waves would do in
+        // registers, so the result is the same
+        wait_for_wave(sync + lflag(g, j + 1, j - 1), epoch, ctl);
+        const double* P = Lt + (size_t)tix(j + 1, j - 1) * TT;
+        const double* Q = Lt + (size_t)tix(j, j - 1) * TT;
+        for (int t = 0; t < (wv == 1 ? 2 : 1); ++t) {
+            const int pi = wv == 1 ? t : wv - 2, pj = wv == 1 ? t : 3 - wv;
+            dbl4 c = quad_load(C1, LDT, pi, pj, lane);
+            tile_mma<true>(c, P, TB, Q, TB, -1.0, pi, pj, lane);
+            quad_store(c, C1, LDT, pi, pj, lane);
+        }
     }
     __syncthreads();
     if (has1) {
-        // L(j+1,j) = C(j+1,j) D(j)^T after C(j+1,j)'s last update
-        if (j > 0) {
-            wait_for(sync + lflag(g, j + 1, j - 1), epoch, ctl);
-            tile_mma<true>(acc1, Lt + (size_t)tix(j + 1, j - 1) * TT, TB,
-                           Lt + (size_t)tix(j, j - 1) * TT, TB, -1.0, qi, qj, lane);
-        }
-        double* C1 = Ws + 16 * 17;  // 32 x 33
-        quad_store(acc1, C1, LDT, qi, qj, lane);
-        __syncthreads();
+        // L(j+1,j) = C(j+1,j) D(j)^T
         dbl4 l1 = dbl4{0, 0, 0, 0};
         tile_mma<true>(l1, C1, LDT, Ds, LDT, 1.0, qi, qj, lane);
         quad_store(l1, Lt + (size_t)tix(j + 1, j) * TT, TB, qi, qj, lane);

@@ -166,6 +166,29 @@ __device__ void wait_for_wave(const int* flag, int epoch, int* ctl) {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 }
 
+// wait_for over a set of n flags that are published at about the same time:
+// the lanes of wave 0 poll flag_of(0 .. n-1) at once, then one acquire --
+// one polling round trip for the set instead of one per flag.
+template <typename FlagOf>
+__device__ void wait_all(FlagOf flag_of, int n, int epoch, int* ctl) {
+    if (threadIdx.x < 64) {
+        for (int l = threadIdx.x; l < n; l += 64) {
+            const int* f = flag_of(l);
+            int cnt = 0;
+            while ((int)(ld_flag(f) - epoch) < 0) {
+                ++cnt;
+                if (cnt > kSpinLimit || ((cnt & 1023) == 0 && ld_flag(ctl + 1) != 0)) {
+                    st_flag(ctl + 1, 1);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    __syncthreads();
+}
+
 // Every wave waits for its own stores to reach the L2 (vmcnt 0); after the
 // barrier one lane's agent-scope release fence writes the L2 back -- the
 // workgroup's waves share one CU and so one L2 -- and raises the flag.
@@ -572,8 +595,11 @@ __device__ void task_row(const StepArgs& a, const Layout& g, int i, int epoch) {
     const double* Xt = a.ws + g.oX;
     const int r = tid >> 3, q = tid & 7;
     double s = 0.0;
+    // every X(i, .) tile follows D(i) within a hand-off or two: wait for the
+    // whole row at once
+    wait_all([&](int j) { return j == i ? sync + g.fD + i : sync + g.fX + xid(T, i, j); }, i + 1,
+             epoch, ctl);
     for (int j = 0; j <= i; ++j) {
-        wait_for(j == i ? sync + g.fD + i : sync + g.fX + xid(T, i, j), epoch, ctl);
         const double* X = Xt + (size_t)tix(i, j) * TT + r * TB + 4 * q;
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
@@ -660,8 +686,9 @@ __device__ void task_final(const StepArgs& a, const Layout& g, int epoch) {
     const int tid = threadIdx.x, T = g.T, k = a.k;
     int* sync = a.sync;
     int* ctl = sync + g.ctl;
-    for (int j = 0; j < T; ++j) wait_for(sync + g.fC + j, epoch, ctl);
-    wait_for(sync + g.fF, epoch, ctl);
+    // the column tasks all finish after the last row: wait for them (and the
+    // filter) at once
+    wait_all([&](int j) { return j < T ? sync + g.fC + j : sync + g.fF; }, T + 1, epoch, ctl);
     if (tid < 4) {
         const int p = tid;
         double P[7] = {0, 0, 0, 0, 0, 0, 0};

@@ -633,7 +633,7 @@ def main():
     args = p.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
-              "design is described in DESIGN.md but not built.", file=sys.stderr)
+              "path is a design only; the figures are for one GPU.", file=sys.stderr)
         sys.exit(2)
     if args.roofline_only:
         roofline_only(args)

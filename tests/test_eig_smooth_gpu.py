@@ -321,3 +321,32 @@ def test_per_column_degree_filter_equals_uniform_filters_per_group(dtype):
         err = float(((Y[:, sl] - ref).abs().max() / ref.abs().max()).item())
         assert err < tol, (grp, err)
 
+
+
+def test_binary16_filter_pass_is_the_fp32_filter_up_to_column_scale():
+    """_filter_pass(half=True) scales each column to unit RMS, stores the block
+    as binary16 and normalises the Chebyshev filter at 0 instead of a0: the
+    same polynomial of A up to a constant per column.  So every output column
+    must be parallel to the FP32 pass's column (sine of the angle within a few
+    binary16 roundings per step: 2e-2 at degree 13), on the b = 2 ring with
+    a0 = 0.5 above the lowest eigenvalues, where the FP32 filter amplifies."""
+    import torch
+    from rvgp_amd.eigsolve import _filter_pass
+    n, theta = 400, 0.7
+    rp, c, blocks = ring_connection_laplacian(n, theta)
+    A = _bsr_from_blocks(rp, c, blocks, 2)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(11)
+    X = torch.randn((2 * n, 48), generator=g, device="cuda", dtype=torch.float64).float()
+    ms = np.repeat([4, 4, 9, 10, 10, 13], 8)
+    a, b, a0 = 0.05, 4.0, 0.5
+    Y32 = _filter_pass(A, X.clone(), ms, 13, a, b, a0, half=False).double()
+    Y16 = _filter_pass(A, X.clone(), ms, 13, a, b, a0, half=True)
+    assert Y16.dtype == torch.float32 and torch.isfinite(Y16).all()
+    Y16 = Y16.double()
+    cos = (Y32 * Y16).sum(0).abs() / (Y32.norm(dim=0) * Y16.norm(dim=0))
+    sin = torch.sqrt(torch.clamp(1.0 - cos * cos, min=0.0))
+    assert float(sin.max()) < 2e-2, sin.max()
+    # and the FP32 pass itself grows the directions below a0 (what the
+    # normalisation at 0 keeps out of binary16's range)
+    assert float(Y32.norm(dim=0).max()) > float(X.double().norm(dim=0).max())

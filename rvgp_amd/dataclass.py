@@ -72,7 +72,7 @@ class data:
         sync()
         t["laplacian_L"] = time.perf_counter() - t0
         resL = {}
-        side = torch.cuda.Stream(device=main.device)
+        side = _side_stream(main.device)
         side.wait_stream(main)
 
         def solve_scalar():
@@ -335,6 +335,25 @@ class data:
         else:
             raise NotImplementedError
         self.vectors = self._frame(1, c, self._X.shape[1])
+
+
+_SIDE_STREAMS = {}
+_SIDE_LOCK = threading.Lock()
+
+
+def _side_stream(device):
+    """The second (default-priority) stream of a device, created once and
+    reused by every data object: the caching allocator keeps its blocks per
+    stream, so a fresh stream per data object found no cached blocks and sent
+    the scalar spectrum's first allocations to hipMalloc, which can wait for
+    the other stream's queued work.  Data objects built concurrently from
+    several host threads on one device share it (their scalar spectra then
+    run one after the other)."""
+    key = torch.device(device).index
+    with _SIDE_LOCK:
+        if key not in _SIDE_STREAMS:
+            _SIDE_STREAMS[key] = torch.cuda.Stream(device=device)
+        return _SIDE_STREAMS[key]
 
 
 def _quiet_dimension(Sigma, frac):

@@ -27,13 +27,13 @@ namespace rvgp {
 // and centred (pyx:397-404): the mean of each coordinate summed in pop order.
 // G[p * rs + q * cs] (rs = Kp1, cs = 1: row-major).
 __device__ void gauges_load_centred(const double* __restrict__ x, int D, const int* idx,
-                                    int Kp1, double* G, int lane, int rs, int cs) {
-    for (int e = lane; e < D * Kp1; e += 64) {
+                                    int Kp1, double* G, int lane, int rs, int cs, int nl = 64) {
+    for (int e = lane; e < D * Kp1; e += nl) {
         int q = e / D, p = e - q * D;
         G[p * rs + q * cs] = x[(size_t)idx[q] * D + p];
     }
     __syncthreads();
-    for (int p = lane; p < D; p += 64) {
+    for (int p = lane; p < D; p += nl) {
         double mean = 0.0;
         for (int q = 0; q < Kp1; ++q) mean += G[p * rs + q * cs];
         mean = mean / Kp1;
@@ -243,12 +243,13 @@ static size_t gauges_smem(int D, int Kp1) { return GaugesLds(D, Kp1).bytes(Kp1);
 // lanes agree on every rotation).  Padding columns and rows are zero.
 struct OneSidedPlan {
     int m2, half, lp, rpl, ldg;
-    __host__ __device__ OneSidedPlan(int D, int Kp1) {
+    // nl: the lanes one neighbourhood gets (64, or 32 when a wave takes two)
+    __host__ __device__ OneSidedPlan(int D, int Kp1, int nl = 64) {
         m2 = Kp1 + (Kp1 & 1);
         half = m2 / 2;
         int hp = 1;
         while (hp < half) hp <<= 1;
-        lp = 64 / hp;                     // lanes per pair (>= 2: K+1 < D <= 64)
+        lp = nl / hp;                     // lanes per pair (>= 2 for nl = 64: K+1 < D <= 64)
         const int r = (D + lp - 1) / lp;  // rows per lane
         rpl = r <= 4 ? 4 : r <= 8 ? 8 : r <= 16 ? 16 : 32;
         // +2 doubles: consecutive columns start 16 bytes x (odd) apart, so
@@ -258,33 +259,55 @@ struct OneSidedPlan {
     __host__ __device__ size_t bytes(int Kp1) const {
         return sizeof(double) * ((size_t)m2 * ldg + m2) + sizeof(int) * ((size_t)Kp1 + m2);
     }
+    // one neighbourhood's LDS in doubles, rounded up to 16 bytes (the next
+    // neighbourhood's columns stay 16-byte aligned)
+    __host__ __device__ size_t node_doubles(int Kp1) const { return (bytes(Kp1) + 15) / 16 * 2; }
+    // two neighbourhoods per wave: every pair keeps >= 2 lanes and <= 32 rows a lane
+    __host__ __device__ bool fits(int D) const { return lp >= 2 && (D + lp - 1) / lp <= 32; }
 };
 
-template <int RPL>
+// sum over the nl lanes of one neighbourhood (nl = 64: the whole wave)
+template <int NL>
+__device__ __forceinline__ double node_sum(double v) {
+    if (NL == 64) return wave_sum(v);
+#pragma unroll
+    for (int o = NL / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// NPW neighbourhoods per wave (1, or 2 when the plan for 32 lanes fits):
+// each gets NL = 64 / NPW lanes and its own LDS region; a step's fixed cost
+// (the rotation from the three sums, the shuffles, the barrier) is then
+// shared by NPW neighbourhoods.  With two, the wave sweeps until neither
+// rotates: a converged neighbourhood's pairs stay below the rotation
+// threshold, so its columns do not change meanwhile.
+template <int RPL, int NPW>
 __global__ __launch_bounds__(64) void gauges_onesided_kernel(
     const double* __restrict__ x, int n, int D, const int* __restrict__ nbh, int Kp1, int ncols,
     int nsv, int n_check, double* __restrict__ tangents, double* __restrict__ sigma,
     int* __restrict__ status) {
     extern __shared__ double smem[];
-    const int i = blockIdx.x;
-    const int lane = threadIdx.x;
-    const OneSidedPlan P(D, Kp1);
+    constexpr int NL = 64 / NPW;
+    const int nd = threadIdx.x / NL, lane = threadIdx.x - nd * NL;
+    const int i = blockIdx.x * NPW + nd;
+    const bool live = i < n;  // a missing second neighbourhood reads node 0's, writes nothing
+    const OneSidedPlan P(D, Kp1, NL);
     const int m2 = P.m2, half = P.half, lp = P.lp, ldg = P.ldg;
-    double* Gc = smem;                                       // column c at Gc + c * ldg
+    double* Gc = smem + nd * P.node_doubles(Kp1);            // column c at Gc + c * ldg
     double* sig = Gc + (size_t)m2 * ldg;                     // m2 column norms
     int* idx = reinterpret_cast<int*>(sig + m2);             // Kp1
     int* order = idx + Kp1;                                  // m2
 
-    for (int e = lane; e < m2 * ldg; e += 64) Gc[e] = 0.0;
-    for (int q = lane; q < Kp1; q += 64) idx[q] = nbh[(size_t)i * Kp1 + q];
+    for (int e = lane; e < m2 * ldg; e += NL) Gc[e] = 0.0;
+    for (int q = lane; q < Kp1; q += NL) idx[q] = nbh[(size_t)(live ? i : 0) * Kp1 + q];
     __syncthreads();
-    gauges_load_centred(x, D, idx, Kp1, Gc, lane, 1, ldg);  // pyx:397-404
+    gauges_load_centred(x, D, idx, Kp1, Gc, lane, 1, ldg, NL);  // pyx:397-404
 
     // columns of norm^2 below 1e-28 |G|_F^2 are numerically zero (the
     // centring leaves at least one): their rotations would only stir noise
     double fro = 0.0;
-    for (int e = lane; e < m2 * ldg; e += 64) fro = __fma_rn(Gc[e], Gc[e], fro);
-    fro = wave_sum(fro);
+    for (int e = lane; e < m2 * ldg; e += NL) fro = __fma_rn(Gc[e], Gc[e], fro);
+    fro = node_sum<NL>(fro);
     const double floor2 = 1e-28 * fro;
 
     const int k = lane / lp, j = lane - k * lp;   // pair, row block
@@ -355,13 +378,13 @@ __global__ __launch_bounds__(64) void gauges_onesided_kernel(
         if (!any_rot) break;
     }
     // singular values: column norms; descending, ties lower index first
-    for (int c = lane; c < m2; c += 64) {
+    for (int c = lane; c < m2; c += NL) {
         double s2 = 0.0;
         for (int r = 0; r < D; ++r) s2 = __fma_rn(Gc[c * ldg + r], Gc[c * ldg + r], s2);
         sig[c] = sqrt(s2);
     }
     __syncthreads();
-    for (int a = lane; a < Kp1; a += 64) {
+    for (int a = lane; a < Kp1; a += NL) {
         const double la = sig[a];
         int rank = 0;
         for (int b = 0; b < Kp1; ++b) {
@@ -372,18 +395,20 @@ __global__ __launch_bounds__(64) void gauges_onesided_kernel(
     }
     __syncthreads();
     int bad = 0;
-    for (int r = lane; r < Kp1; r += 64) {
-        const double sv = sig[order[r]];
-        if (r < nsv) sigma[(size_t)i * nsv + r] = sv;
-        if (r < n_check && sv < 1e-10) bad = 1;
+    if (live) {
+        for (int r = lane; r < Kp1; r += NL) {
+            const double sv = sig[order[r]];
+            if (r < nsv) sigma[(size_t)i * nsv + r] = sv;
+            if (r < n_check && sv < 1e-10) bad = 1;
+        }
+        for (int e = lane; e < D * ncols; e += NL) {
+            const int p = e / ncols, r = e - p * ncols;
+            const int c = order[r];
+            const double sv = sig[c];
+            tangents[(size_t)i * D * ncols + e] = sv > 0.0 ? Gc[c * ldg + p] / sv : 0.0;
+        }
     }
-    for (int e = lane; e < D * ncols; e += 64) {
-        const int p = e / ncols, r = e - p * ncols;
-        const int c = order[r];
-        const double sv = sig[c];
-        tangents[(size_t)i * D * ncols + e] = sv > 0.0 ? Gc[c * ldg + p] / sv : 0.0;
-    }
-    if (__any(bad) && lane == 0) atomicMax(status, RVGP_ST_RANK_DEFICIENT);
+    if (__any(bad) && threadIdx.x == 0) atomicMax(status, RVGP_ST_RANK_DEFICIENT);
 }
 
 }  // namespace rvgp
@@ -400,16 +425,24 @@ extern "C" int rvgp_gauges(const double* x, int n, int D, const int* nbh, int Kp
                    "rvgp_gauges: ncols/nsv/n_check must not exceed min(D, K+1)");
     hipStream_t st = as_stream(stream);
     if (D > Kp1) {
-        const OneSidedPlan P(D, Kp1);
-        const size_t sm = P.bytes(Kp1);
+        // two neighbourhoods per wave where the 32-lane plan fits (K+1 <= 32)
+        const OneSidedPlan P2(D, Kp1, 32);
+        const int npw = P2.fits(D) ? 2 : 1;
+        const OneSidedPlan P = npw == 2 ? P2 : OneSidedPlan(D, Kp1);
+        const size_t sm = npw == 2 ? 2 * sizeof(double) * P.node_doubles(Kp1) : P.bytes(Kp1);
         RVGP_CHECK_ARG(sm <= 65536, "rvgp_gauges: D=%d, K+1=%d needs %zu bytes of LDS", D, Kp1, sm);
-#define RVGP_ONESIDED(R)                                                                     \
-    hipLaunchKernelGGL(gauges_onesided_kernel<R>, dim3(n), dim3(64), sm, st, x, n, D, nbh, Kp1, \
+        const dim3 grid((unsigned)cdiv(n, npw));
+#define RVGP_ONESIDED(R, W)                                                                    \
+    hipLaunchKernelGGL((gauges_onesided_kernel<R, W>), grid, dim3(64), sm, st, x, n, D, nbh, Kp1, \
                        ncols, nsv, n_check, tangents, sigma, status)
-        if (P.rpl == 4) RVGP_ONESIDED(4);
-        else if (P.rpl == 8) RVGP_ONESIDED(8);
-        else if (P.rpl == 16) RVGP_ONESIDED(16);
-        else RVGP_ONESIDED(32);
+#define RVGP_ONESIDED_W(W)                       \
+    if (P.rpl == 4) RVGP_ONESIDED(4, W);         \
+    else if (P.rpl == 8) RVGP_ONESIDED(8, W);    \
+    else if (P.rpl == 16) RVGP_ONESIDED(16, W);  \
+    else RVGP_ONESIDED(32, W);
+        if (npw == 2) { RVGP_ONESIDED_W(2) }
+        else { RVGP_ONESIDED_W(1) }
+#undef RVGP_ONESIDED_W
 #undef RVGP_ONESIDED
         RVGP_LAUNCH_CHECK();
         return RVGP_OK;
